@@ -1,0 +1,90 @@
+"""ctypes binding of libgalacticus_amd.so (the C ABI of include/gal_ode.h).
+
+The library must have been built (``python -m galacticus_amd.build`` or
+``__graft_entry__.build()``); loading fails loudly otherwise -- there is no
+CPU fallback in the product.
+"""
+from __future__ import annotations
+
+import ctypes
+import os
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(HERE, "lib", "libgalacticus_amd.so")
+
+GAL_OK = 0
+ERRORS = {-1: "GAL_E_INVALID", -2: "GAL_E_HIP", -3: "GAL_E_NOMEM", -4: "GAL_E_NODEVICE"}
+
+# Symbols declared in include/gal_ode.h and include/gal_surrogate.h.
+EXPORTS = [
+    "gal_ode_version", "gal_ode_device_check", "gal_ode_batch_create", "gal_ode_batch_destroy",
+    "gal_ode_batch_set_physics", "gal_ode_batch_set_cooling_table", "gal_ode_batch_set_launch",
+    "gal_ode_batch_evolve_d", "gal_ode_batch_evolve_h", "gal_ode_rhs_d", "gal_ode_hadjust_d",
+    "gal_ode_batch_autotune",
+]
+
+_lib = None
+
+
+class GalError(RuntimeError):
+    pass
+
+
+def check(rc: int, what: str):
+    if rc != GAL_OK:
+        raise GalError(f"{what} returned {ERRORS.get(rc, rc)}")
+
+
+def load():
+    """Load the HIP library; raises if it has not been built."""
+    global _lib
+    if _lib is not None:
+        return _lib
+    if not os.path.exists(LIB_PATH):
+        raise GalError(f"{LIB_PATH} is missing: build the HIP library first (python -m galacticus_amd.build)")
+    L = ctypes.CDLL(LIB_PATH)
+    vp, dp, ip, i64p = ctypes.c_void_p, ctypes.POINTER(ctypes.c_double), ctypes.POINTER(ctypes.c_int), ctypes.c_void_p
+    c_int, c_double = ctypes.c_int, ctypes.c_double
+    L.gal_ode_version.restype = c_int
+    L.gal_ode_device_check.argtypes = [ip, ip]
+    L.gal_ode_batch_create.argtypes = [c_int, c_double, c_double, c_int, ctypes.POINTER(vp)]
+    L.gal_ode_batch_destroy.argtypes = [vp]
+    L.gal_ode_batch_set_physics.argtypes = [vp, dp, c_int]
+    L.gal_ode_batch_set_cooling_table.argtypes = [vp, dp, c_int, c_double, c_double, c_int, c_double, c_double]
+    L.gal_ode_batch_set_launch.argtypes = [vp, c_int, c_int]
+    L.gal_ode_batch_evolve_d.argtypes = [vp, c_int, c_int, vp, vp, vp, vp, vp, vp, vp, vp, vp]
+    L.gal_ode_batch_evolve_h.argtypes = [vp, c_int, dp, dp, dp, dp, dp, dp, ip, i64p]
+    L.gal_ode_rhs_d.argtypes = [vp, c_int, c_int, vp, vp, vp, vp, vp]
+    L.gal_ode_hadjust_d.argtypes = [c_int, c_int, c_int, c_double, c_double, c_double, c_double, vp, vp, vp, vp,
+                                    vp, vp, vp, vp]
+    L.gal_ode_batch_autotune.argtypes = [vp, c_int, c_int, vp, vp, vp, vp, vp, vp, vp, vp, ip, ip]
+    for name in EXPORTS:
+        getattr(L, name).restype = c_int
+    _load_surrogate(L)
+    _lib = L
+    return L
+
+
+SURROGATE_EXPORTS = [
+    "gal_mlp_create", "gal_mlp_destroy", "gal_mlp_set_weights", "gal_mlp_get_weights", "gal_mlp_forward_d",
+    "gal_mlp_train_step_d", "gal_mlp_grad_d", "gal_surrogate_step_d", "gal_mlp_param_count",
+]
+
+
+def _load_surrogate(L):
+    vp, c_int, c_float, c_double = ctypes.c_void_p, ctypes.c_int, ctypes.c_float, ctypes.c_double
+    if not hasattr(L, "gal_mlp_create"):
+        return
+    L.gal_mlp_param_count.argtypes = []
+    L.gal_mlp_create.argtypes = [c_int, ctypes.POINTER(vp)]
+    L.gal_mlp_destroy.argtypes = [vp]
+    L.gal_mlp_set_weights.argtypes = [vp, vp]
+    L.gal_mlp_get_weights.argtypes = [vp, vp]
+    L.gal_mlp_forward_d.argtypes = [vp, c_int, vp, vp, vp]
+    L.gal_mlp_grad_d.argtypes = [vp, c_int, vp, vp, vp, vp, vp]
+    L.gal_mlp_train_step_d.argtypes = [vp, c_int, vp, vp, vp, c_float, c_float, c_float, c_float, c_float, c_int,
+                                       vp, vp]
+    L.gal_surrogate_step_d.argtypes = [vp, vp, c_int, c_int, vp, vp, vp, vp, vp, vp, vp, c_double, c_double, vp,
+                                       vp, vp]
+    for name in SURROGATE_EXPORTS:
+        getattr(L, name).restype = c_int

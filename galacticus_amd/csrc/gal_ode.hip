@@ -1,0 +1,793 @@
+// gal_ode.hip -- batched node-ODE evolution on MI355X (gfx950) and its C ABI.
+//
+// One lane owns one node.  A persistent grid of workgroups claims nodes from
+// a global counter (one atomic per wave per refill) and integrates each from
+// t_start to t_end with the Cash-Karp (4,5) pair, the scaled error norm and
+// step controller, the driver loop and the node-evolver trial loop of the
+// reference:
+//   trial loop    node_evolver/standard.F90:587-740
+//   solve         numerical/ODE_solver/solver.F90:492-691
+//   driver        external/gslODEInitVal2/driver2.c:148-250
+//   evolve/step   GSL 2.6 gsl_odeiv2_evolve_apply + rkck (published method)
+//   controller    external/gslODEInitVal2/cscal2.c:93-169
+// When a lane's node finishes it claims the next one, so lanes stay busy
+// while step counts differ by two orders of magnitude between nodes.
+//
+// Schedule.  The lanes of a wave advance in lockstep through a six-phase
+// loop, one Cash-Karp attempt per trip:
+//   phases 0-4  stage inputs for k2..k6, RHS, store (phase 4: combine,
+//               error norm, controller, accept / reject / fail)
+//   phase 5     claim nodes for lanes that finished, then one shared RHS
+//               slot at (t, y): dydt at the new point (the reference's
+//               dydt_out, reused as the next k1), or k1 of a freshly claimed
+//               node, or k1 after a trial restart.
+// There is a single RHS call site, every lane spends six RHS evaluations per
+// trip whatever it is doing, and the RHS never diverges by phase.
+//
+// Memory.  State is structure-of-arrays in HBM (y[v*ld + i]); it is read once
+// when a node is claimed and written once when it finishes.  Within a node,
+// the RHS input and output stay in registers, and the five stage
+// derivatives k1..k5, the state y and the error scales live in LDS in a
+// lane-private, bank-conflict-free layout L[(vec*30 + v)*BLOCK + lane]
+// (7*30*8 = 1680 bytes per lane, 105 KiB per 64-lane workgroup).  Nothing
+// is addressable private memory, so the long dependent chain of a node never
+// waits on scratch.
+#include <hip/hip_runtime.h>
+#include <cstdint>
+#include <cstdio>
+#include <cstring>
+#include <cmath>
+#include <algorithm>
+
+#include "../../include/gal_ode.h"
+#include "constants.hpp"
+#include "physics.hpp"
+#include "integrator.hpp"
+
+using namespace gal;
+
+namespace {
+
+constexpr int kStages = 5;  // k1..k5 kept; k6 is consumed as it is produced
+#ifndef GAL_SC_IN_LDS
+#define GAL_SC_IN_LDS 1
+#endif
+constexpr int kLdsVectors = kStages + 1 + GAL_SC_IN_LDS;
+
+struct EvolveArgs {
+  int n, ld;
+  double* y;
+  const double* np;
+  const double* t_start;
+  const double* t_end;
+  double* h_step;
+  double* scale;
+  int* status;
+  long long* counters;
+  const int* order;
+  unsigned int* work;
+  const double* P;
+  const double* tab;
+  DerivedConstants D;
+  CoolingDesc cd;
+  double eps_abs, eps_rel;
+  int flags, mode;
+};
+
+__device__ __forceinline__ NodePar load_node_par(const double* __restrict__ np, int ld, int i) {
+  NodePar p;
+  p.t_ref = np[GAL_NP_TIME_REFERENCE * (size_t)ld + i];
+  p.m_ref = np[GAL_NP_MASS_REFERENCE * (size_t)ld + i];
+  p.m_rate = np[GAL_NP_MASS_RATE * (size_t)ld + i];
+  p.spin_rate = np[GAL_NP_SPIN_RATE * (size_t)ld + i];
+  p.satellite = np[GAL_NP_IS_SATELLITE * (size_t)ld + i] != 0.0;
+  return p;
+}
+
+// isNonNegative attributes of the state variables (component _class.F90 XML).
+__device__ __forceinline__ bool var_non_negative(int v) {
+  return !(v == GAL_V_HOT_ANGULAR_MOMENTUM || v == GAL_V_HOT_OUTFLOWED_ANGULAR_MOMENTUM ||
+           v == GAL_V_HOT_ENERGY_RADIATED || v == GAL_V_BH_RADIAL_POSITION || v == GAL_V_SPIN_ANGULAR_MOMENTUM);
+}
+
+// Claim one node index per requesting lane with a single atomic per wave.
+__device__ __forceinline__ int wave_claim(bool want, unsigned int* work, int n, const int* order) {
+  const unsigned long long mask = __ballot(want);
+  int idx = -1;
+  if (mask) {
+    const int lane = __lane_id();
+    const int leader = __ffsll((long long)mask) - 1;
+    const int cnt = __popcll(mask);
+    unsigned int base = 0;
+    if (lane == leader) base = atomicAdd(work, (unsigned int)cnt);
+    base = __shfl(base, leader);
+    const unsigned long long below = lane ? (mask & ((1ull << lane) - 1ull)) : 0ull;
+    const unsigned int mine = base + (unsigned int)__popcll(below);
+    if (want && mine < (unsigned int)n) idx = order ? order[mine] : (int)mine;
+  }
+  return idx;
+}
+
+__device__ __forceinline__ void write_counters(const EvolveArgs& a, int node, long long na, long long nc,
+                                               long long nr) {
+  if (a.counters) {
+    a.counters[node] = na;
+    a.counters[(size_t)a.n + node] = nc;
+    a.counters[2 * (size_t)a.n + node] = nr;
+  }
+}
+
+template <int BLOCK>
+__global__ __launch_bounds__(BLOCK, 1) void evolve_kernel(EvolveArgs a) {
+  extern __shared__ double lds_k[];
+  double* __restrict__ K = lds_k + threadIdx.x;   // K[(s*kNV + v)*BLOCK]
+  const double* __restrict__ P = a.P;
+  const int mode = a.mode;
+  const bool reuse = (a.flags & GAL_FLAG_REUSE_STEP_SIZE) != 0;
+  const bool enforce = (a.flags & GAL_FLAG_ENFORCE_NON_NEGATIVITY) != 0;
+  const bool scales_given = (a.flags & GAL_FLAG_SCALES_PROVIDED) != 0;
+#define KS(s, v) K[((s) * kNV + (v)) * BLOCK]
+
+  double* __restrict__ YL = lds_k + kStages * kNV * BLOCK + threadIdx.x;
+#define Y(v) YL[(v) * BLOCK]
+#if GAL_SC_IN_LDS
+  double* __restrict__ SL = lds_k + (kStages + 1) * kNV * BLOCK + threadIdx.x;
+#define SC(v) SL[(v) * BLOCK]
+#else
+  double sc[kNV];
+#define SC(v) sc[v]
+#endif
+  double ytmp[kNV], kout[kNV];
+  NodePar np{};
+  int node = -1;
+  bool active = false, want = true, eval_pending = false, post_pending = false, final_step = false;
+  double t = 0.0, t0 = 0.0, t1 = 0.0, h = 0.0, h0 = 0.0, h_node = 0.0, sign = 1.0;
+  int trial = 0;
+  long long n_attempt = 0, n_accept = 0, n_rhs = 0;
+
+  int phase = 5;
+  for (;;) {
+    bool do_rhs = false;
+    double ts = 0.0;
+    if (phase == 5) {
+      // ---- claim nodes for lanes that need one (convergent point) ------
+      while (__any(want)) {
+        const int idx = wave_claim(want, a.work, a.n, a.order);
+        if (want) {
+          if (idx < 0) {
+            want = false;
+            active = false;
+          } else {
+            node = idx;
+            np = load_node_par(a.np, a.ld, node);
+            t0 = a.t_start[node];
+            t1 = a.t_end[node];
+            h_node = a.h_step[node];
+#pragma unroll
+            for (int v = 0; v < kNV; ++v) ytmp[v] = a.y[v * (size_t)a.ld + node];
+            if (scales_given) {
+#pragma unroll
+              for (int v = 0; v < kNV; ++v) kout[v] = a.scale[v * (size_t)a.ld + node];
+            } else if (mode == GAL_PHYSICS_TEST) {
+#pragma unroll
+              for (int v = 0; v < kNV; ++v) kout[v] = 1.0;
+            } else {
+              scales(P, a.D, np, t0, ytmp, kout);
+            }
+            if (!scales_given && a.scale) {
+#pragma unroll
+              for (int v = 0; v < kNV; ++v) a.scale[v * (size_t)a.ld + node] = kout[v];
+            }
+#pragma unroll
+            for (int v = 0; v < kNV; ++v) {
+              Y(v) = ytmp[v];
+              SC(v) = kout[v];
+            }
+            n_attempt = n_accept = n_rhs = 0;
+            trial = 0;
+            if (t0 == t1) {
+              // nothing to integrate (standard.F90:580): success, timeStep -1
+              a.h_step[node] = -1.0;
+              a.status[node] = GAL_STATUS_SUCCESS;
+              write_counters(a, node, 0, 0, 0);
+            } else {
+              want = false;
+              active = true;
+              // odeSolverSolve initial step (solver.F90:537-555)
+              const double step_size = reuse ? h_node : -1.0;
+              double xs = t1 - t0;
+              if (step_size > 0.0) xs = fmin(step_size, xs);
+              h = xs;
+              h0 = h;
+              sign = h > 0.0 ? 1.0 : -1.0;
+              t = t0;
+              eval_pending = true;
+            }
+          }
+        }
+      }
+      if (eval_pending) {
+        do_rhs = true;
+        ts = t;
+#pragma unroll
+        for (int v = 0; v < kNV; ++v) ytmp[v] = Y(v);
+      }
+    } else if (active) {
+      do_rhs = true;
+      switch (phase) {
+        case 0: {
+          // gsl_odeiv2_evolve_apply: clip the step to reach t1 exactly
+          const double dt = t1 - t;
+          final_step = false;
+          if ((dt >= 0.0 && h0 > dt) || (dt < 0.0 && h0 < dt)) {
+            h0 = dt;
+            final_step = true;
+          }
+          ++n_attempt;
+          ts = t + ck::a2 * h0;
+#pragma unroll
+          for (int v = 0; v < kNV; ++v) ytmp[v] = Y(v) + h0 * (ck::b21 * KS(0, v));
+          break;
+        }
+        case 1:
+          ts = t + ck::a3 * h0;
+#pragma unroll
+          for (int v = 0; v < kNV; ++v) ytmp[v] = Y(v) + h0 * (ck::b31 * KS(0, v) + ck::b32 * KS(1, v));
+          break;
+        case 2:
+          ts = t + ck::a4 * h0;
+#pragma unroll
+          for (int v = 0; v < kNV; ++v)
+            ytmp[v] = Y(v) + h0 * (ck::b41 * KS(0, v) + ck::b42 * KS(1, v) + ck::b43 * KS(2, v));
+          break;
+        case 3:
+          ts = t + ck::a5 * h0;
+#pragma unroll
+          for (int v = 0; v < kNV; ++v)
+            ytmp[v] = Y(v) + h0 * (ck::b51 * KS(0, v) + ck::b52 * KS(1, v) + ck::b53 * KS(2, v) +
+                                   ck::b54 * KS(3, v));
+          break;
+        default:
+          ts = t + ck::a6 * h0;
+#pragma unroll
+          for (int v = 0; v < kNV; ++v)
+            ytmp[v] = Y(v) + h0 * (ck::b61 * KS(0, v) + ck::b62 * KS(1, v) + ck::b63 * KS(2, v) +
+                                   ck::b64 * KS(3, v) + ck::b65 * KS(4, v));
+          break;
+      }
+    }
+
+    if (do_rhs) {
+      rhs(P, a.D, a.tab, a.cd, mode, np, ts, ytmp, kout);
+      ++n_rhs;
+    }
+
+    if (phase < 4) {
+      if (active) {
+#pragma unroll
+        for (int v = 0; v < kNV; ++v) KS(phase + 1, v) = kout[v];
+      }
+      ++phase;
+      continue;
+    }
+    if (phase == 5) {
+      if (eval_pending) {
+#pragma unroll
+        for (int v = 0; v < kNV; ++v) KS(0, v) = kout[v];
+        eval_pending = false;
+        if (post_pending) {
+          post_step(P, a.D, np, mode, t, &Y(0), BLOCK);   // GSL_CONTINUE: no evolver reset
+          post_pending = false;
+        }
+      }
+      if (!__any(active)) break;
+      phase = 0;
+      continue;
+    }
+
+    // ---- phase 4: k6 is in kout; combine, error norm, controller --------
+    phase = 5;
+    if (!active) continue;
+    ErrorNorm en;
+#pragma unroll
+    for (int v = 0; v < kNV; ++v) {
+      const double k1v = KS(0, v), k3v = KS(2, v), k4v = KS(3, v), k5v = KS(4, v);
+      const double yn = Y(v) + h0 * (ck::c1 * k1v + ck::c3 * k3v + ck::c4 * k4v + ck::c6 * kout[v]);
+      const double ye = h0 * (ck::ec1 * k1v + ck::ec3 * k3v + ck::ec4 * k4v + ck::ec5 * k5v + ck::ec6 * kout[v]);
+      ytmp[v] = yn;
+      error_norm_add(en, a.eps_abs, a.eps_rel, 1.0, 0.0, h0, yn, ye, 0.0, SC(v), enforce && var_non_negative(v));
+    }
+    double h_adj = h0;
+    const int decision = hadjust_decide(en, kRkckOrder, h_adj);
+    const double t_adv = final_step ? t1 : t + h0;
+    bool failed = false;
+    if (decision == kHadjDec) {
+      if (fabs(h_adj) < fabs(h0) && t_adv + h_adj != t_adv) h0 = h_adj;  // reject: retry, same k1
+      else failed = true;                                                 // step size underflow
+    } else {
+      ++n_accept;
+      t = t_adv;
+      if (!final_step) h = h_adj;
+      h0 = h;
+#pragma unroll
+      for (int v = 0; v < kNV; ++v) Y(v) = ytmp[v];
+      if (sign * (t1 - t) > 0.0) {
+        eval_pending = true;   // dydt at the new point, then the post-step
+        post_pending = true;
+      } else {
+        post_step(P, a.D, np, mode, t, &Y(0), BLOCK);
+        // odeSolverSolve returns xStep = driver h (solver.F90:593)
+#pragma unroll
+        for (int v = 0; v < kNV; ++v) a.y[v * (size_t)a.ld + node] = Y(v);
+        a.h_step[node] = h;
+        a.status[node] = GAL_STATUS_SUCCESS;
+        write_counters(a, node, n_attempt, n_accept, n_rhs);
+        active = false;
+        want = true;
+      }
+    }
+    if (failed) {
+      // node-evolver trial loop (standard.F90:657-692)
+      bool rescued = false;
+      if (enforce) {
+        bool any = false;
+#pragma unroll
+        for (int v = 0; v < kNV; ++v) any |= (Y(v) < 0.0 && var_non_negative(v));
+        if (any) {
+#pragma unroll
+          for (int v = 0; v < kNV; ++v)
+            if (Y(v) < 0.0 && var_non_negative(v)) Y(v) = 0.0;
+          rescued = true;
+        }
+      }
+      if (rescued) {
+#pragma unroll
+        for (int v = 0; v < kNV; ++v) a.y[v * (size_t)a.ld + node] = Y(v);
+        a.h_step[node] = h_node / (double)(1 << trial);
+        a.status[node] = GAL_STATUS_SUCCESS;
+        write_counters(a, node, n_attempt, n_accept, n_rhs);
+        active = false;
+        want = true;
+      } else if (++trial < 8) {
+#pragma unroll
+        for (int v = 0; v < kNV; ++v) Y(v) = a.y[v * (size_t)a.ld + node];
+        const double step_size = reuse ? h_node / (double)(1 << trial) : -1.0;
+        double xs = t1 - t0;
+        if (step_size > 0.0) xs = fmin(step_size, xs);
+        h = xs;
+        h0 = h;
+        t = t0;
+        eval_pending = true;
+      } else {
+        // all trials failed: state restored, errorStatusUnderflow (standard.F90:715-718)
+        a.status[node] = GAL_STATUS_UNDERFLOW;
+        write_counters(a, node, n_attempt, n_accept, n_rhs);
+        active = false;
+        want = true;
+      }
+    }
+  }
+#undef KS
+#undef Y
+#undef SC
+}
+
+__global__ void reset_counter_kernel(unsigned int* w) {
+  if (threadIdx.x == 0 && blockIdx.x == 0) *w = 0u;
+}
+
+__global__ void rhs_kernel(int n, int ld, const double* __restrict__ t, const double* __restrict__ y,
+                           const double* __restrict__ npa, double* __restrict__ dydt, const double* __restrict__ P,
+                           const double* __restrict__ tab, DerivedConstants D, CoolingDesc cd, int mode) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  double yy[kNV], f[kNV];
+#pragma unroll
+  for (int v = 0; v < kNV; ++v) yy[v] = y[v * (size_t)ld + i];
+  const NodePar np = load_node_par(npa, ld, i);
+  rhs(P, D, tab, cd, mode, np, t[i], yy, f);
+#pragma unroll
+  for (int v = 0; v < kNV; ++v) dydt[v * (size_t)ld + i] = f[v];
+}
+
+__global__ void hadjust_kernel(int n, int dim, int ord, double eps_abs, double eps_rel, double a_y, double a_dydt,
+                               const double* __restrict__ y, const double* __restrict__ yerr,
+                               const double* __restrict__ yp, const double* __restrict__ scale,
+                               const int* __restrict__ nonneg, double* __restrict__ h, int* __restrict__ result) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  ErrorNorm en;
+  const double h_old = h[i];
+  for (int v = 0; v < dim; ++v) {
+    const size_t k = v * (size_t)n + i;
+    error_norm_add(en, eps_abs, eps_rel, a_y, a_dydt, h_old, y[k], yerr[k], yp[k], scale[k], nonneg[k] != 0);
+  }
+  double hh = h_old;
+  result[i] = hadjust_decide(en, ord, hh);
+  h[i] = hh;
+}
+
+template <int BLOCK>
+constexpr size_t lds_bytes() {
+  return sizeof(double) * kLdsVectors * kNV * BLOCK;
+}
+
+}  // namespace
+
+// ============================================================================
+// C ABI
+// ============================================================================
+struct gal_ode_batch {
+  int n_max = 0;
+  double tol_abs = 0.0, tol_rel = 0.0;
+  int flags = 0;
+  int device = 0;
+  int n_cus = 0;
+  int nodes_per_block = 64;
+  int blocks_per_cu = 1;
+  double params[GAL_NPARAM] = {};
+  DerivedConstants D{};
+  CoolingDesc cd{};
+  double* d_params = nullptr;
+  double* d_table = nullptr;
+  unsigned int* d_work = nullptr;
+  int* d_order = nullptr;
+  bool have_physics = false, have_table = false;
+  // staging for the host-pointer entry point
+  double *d_y = nullptr, *d_np = nullptr, *d_t0 = nullptr, *d_t1 = nullptr, *d_h = nullptr, *d_scale = nullptr;
+  int* d_status = nullptr;
+  long long* d_counters = nullptr;
+};
+
+#define HIP_TRY(x)                                                              \
+  do {                                                                          \
+    hipError_t e_ = (x);                                                        \
+    if (e_ != hipSuccess) {                                                     \
+      fprintf(stderr, "gal_ode: %s failed: %s\n", #x, hipGetErrorString(e_));   \
+      return GAL_E_HIP;                                                         \
+    }                                                                           \
+  } while (0)
+
+static int half_power_code(double alpha) {
+  const double a2 = 2.0 * alpha;
+  if (a2 == std::floor(a2) && std::fabs(a2) <= 16.0) return (int)a2;
+  return kNoHalfPower;
+}
+
+static void derive_constants(gal_ode_batch* h) {
+  const double* p = h->params;
+  DerivedConstants& D = h->D;
+  const double om = p[GAL_P_OMEGA_MATTER], ol = p[GAL_P_OMEGA_DARK_ENERGY], h0 = p[GAL_P_HUBBLE_CONSTANT];
+  D.om = om;
+  D.ol = ol;
+  D.a3_norm = om / ol;
+  D.sinh_arg = 1.5 * std::sqrt(ol) * h0 / kMpcPerKmsToGyr;
+  const double rho_crit0 = 3.0 * h0 * h0 / (8.0 * kPi * kGravity);
+  D.rvir_coeff = 3.0 / (4.0 * kPi * rho_crit0);
+  const double ar = 1.0 / (1.0 + p[GAL_P_REDSHIFT_REIONIZATION]);
+  D.a3_reion = ar * ar * ar;
+  D.f_baryon = p[GAL_P_OMEGA_BARYON] / om;
+  D.inv_200 = 1.0 / 200.0;
+  D.inv_sqrt_200 = 1.0 / std::sqrt(200.0);
+  D.ln_200 = std::log(200.0);
+  D.v_fb_disk = p[GAL_P_FB_DISK_VELOCITY];
+  D.sqrt_v_fb_disk = std::sqrt(D.v_fb_disk);
+  D.ln_v_fb_disk = std::log(D.v_fb_disk);
+  D.v_fb_sph = p[GAL_P_FB_SPHEROID_VELOCITY];
+  D.sqrt_v_fb_sph = std::sqrt(D.v_fb_sph);
+  D.ln_v_fb_sph = std::log(D.v_fb_sph);
+  D.n2_sf_disk = half_power_code(p[GAL_P_SF_DISK_EXPONENT_VELOCITY]);
+  D.n2_sf_sph = half_power_code(p[GAL_P_SF_SPHEROID_EXPONENT_VELOCITY]);
+  D.n2_fb_disk = half_power_code(p[GAL_P_FB_DISK_EXPONENT]);
+  D.n2_fb_sph = half_power_code(p[GAL_P_FB_SPHEROID_EXPONENT]);
+  D.t_vir_per_v2 = 0.5 * kAtomicMassUnit * kMeanAtomicMassPrimordial * kKilo * kKilo / kBoltzmann;
+  D.cs2_per_t = 5.0 / 3.0 * kBoltzmann / (kMeanAtomicMassPrimordial * kAtomicMassUnit) / (kKilo * kKilo);
+  D.cs2_sph = D.cs2_per_t * p[GAL_P_BH_TEMPERATURE_SPHEROID];
+  D.m_edd_per_mass = 4.0 * kPi * kGravitySI * kMassHydrogenAtom * kGigaYear / kThomson / kSpeedLight;
+  const double unit = kErgs * kGigaYear / kMassSolar / (kKilo * kKilo);
+  // (M/1e9)^0.9 (Mdot/(k M))^1.2 = 1e9^-0.9 k^-1.2 M^-0.3 Mdot^1.2
+  const double ln_common = -0.9 * std::log(1.0e9) - 1.2 * std::log(D.m_edd_per_mass);
+  D.ln_jet_kerr = 42.7 * std::log(10.0) + std::log(unit) + ln_common;
+  D.ln_jet_schw = 41.7 * std::log(10.0) + std::log(unit) + ln_common;
+  D.cool_k_pre = p[GAL_P_DEGREES_OF_FREEDOM] / 2.0 * kBoltzmann / kErgs / kGigaYear;
+  D.unit_energy = kMassSolar * kKilo * kKilo / kErgs / kGigaYear;
+  const double mpc_cm = kHecto * kMegaParsec;
+  D.nh_coeff = kMassSolar / kMassHydrogenAtom / (mpc_cm * mpc_cm * mpc_cm);
+}
+
+extern "C" {
+
+int gal_ode_version(void) { return 10000; }
+
+int gal_ode_device_check(int* n_devices, int* n_cus) {
+  int nd = 0;
+  if (hipGetDeviceCount(&nd) != hipSuccess || nd < 1) {
+    if (n_devices) *n_devices = 0;
+    return GAL_E_NODEVICE;
+  }
+  hipDeviceProp_t prop;
+  if (hipGetDeviceProperties(&prop, 0) != hipSuccess) return GAL_E_HIP;
+  if (n_devices) *n_devices = nd;
+  if (n_cus) *n_cus = prop.multiProcessorCount;
+  if (std::strncmp(prop.gcnArchName, "gfx950", 6) != 0) return GAL_E_NODEVICE;
+  return GAL_OK;
+}
+
+int gal_ode_batch_create(int n_max, double tol_abs, double tol_rel, int flags, gal_ode_batch** out) {
+  if (!out || n_max < 0) return GAL_E_INVALID;
+  if (tol_abs < 0.0 || tol_rel < 0.0 || (tol_abs <= 0.0 && tol_rel <= 0.0)) return GAL_E_INVALID;
+  *out = nullptr;
+  int nd = 0, ncu = 0;
+  int rc = gal_ode_device_check(&nd, &ncu);
+  if (rc != GAL_OK) return rc;
+  gal_ode_batch* h = new gal_ode_batch();
+  h->n_max = n_max;
+  h->tol_abs = tol_abs;
+  h->tol_rel = tol_rel;
+  h->flags = flags;
+  if (hipGetDevice(&h->device) != hipSuccess) {
+    delete h;
+    return GAL_E_HIP;
+  }
+  h->n_cus = ncu;
+  const size_t nm = (size_t)std::max(n_max, 1);
+  if (hipMalloc(&h->d_params, sizeof(double) * GAL_NPARAM) != hipSuccess ||
+      hipMalloc(&h->d_table, sizeof(double) * GAL_COOLING_NT_MAX * GAL_COOLING_NZ_MAX) != hipSuccess ||
+      hipMalloc(&h->d_work, sizeof(unsigned int)) != hipSuccess ||
+      hipMalloc(&h->d_order, sizeof(int) * nm) != hipSuccess) {
+    hipFree(h->d_params); hipFree(h->d_table); hipFree(h->d_work); hipFree(h->d_order);
+    delete h;
+    return GAL_E_NOMEM;
+  }
+  static bool attrs_set = false;
+  if (!attrs_set) {
+    static_assert(lds_bytes<64>() <= 163840, "lane-private LDS vectors must fit 160 KiB");
+    if (hipFuncSetAttribute((const void*)evolve_kernel<64>, hipFuncAttributeMaxDynamicSharedMemorySize,
+                            (int)lds_bytes<64>()) != hipSuccess) {
+      (void)hipGetLastError();
+      hipFree(h->d_params); hipFree(h->d_table); hipFree(h->d_work); hipFree(h->d_order);
+      delete h;
+      return GAL_E_HIP;
+    }
+    attrs_set = true;
+  }
+  *out = h;
+  return GAL_OK;
+}
+
+static void free_staging(gal_ode_batch* h) {
+  hipFree(h->d_y); hipFree(h->d_np); hipFree(h->d_t0); hipFree(h->d_t1); hipFree(h->d_h);
+  hipFree(h->d_scale); hipFree(h->d_status); hipFree(h->d_counters);
+  h->d_y = h->d_np = h->d_t0 = h->d_t1 = h->d_h = h->d_scale = nullptr;
+  h->d_status = nullptr;
+  h->d_counters = nullptr;
+}
+
+int gal_ode_batch_destroy(gal_ode_batch* h) {
+  if (!h) return GAL_E_INVALID;
+  hipFree(h->d_params);
+  hipFree(h->d_table);
+  hipFree(h->d_work);
+  hipFree(h->d_order);
+  free_staging(h);
+  delete h;
+  return GAL_OK;
+}
+
+int gal_ode_batch_set_physics(gal_ode_batch* h, const double* params, int n_params) {
+  if (!h || !params || n_params != GAL_NPARAM) return GAL_E_INVALID;
+  const int mode = (int)params[GAL_P_PHYSICS_MODE];
+  if (mode < GAL_PHYSICS_STANDARD || mode > GAL_PHYSICS_TEST) return GAL_E_INVALID;
+  if (!(params[GAL_P_OMEGA_MATTER] > 0.0) || !(params[GAL_P_OMEGA_DARK_ENERGY] > 0.0) ||
+      !(params[GAL_P_HUBBLE_CONSTANT] > 0.0) || !(params[GAL_P_FB_DISK_VELOCITY] > 0.0) ||
+      !(params[GAL_P_FB_SPHEROID_VELOCITY] > 0.0))
+    return GAL_E_INVALID;
+  std::memcpy(h->params, params, sizeof(double) * GAL_NPARAM);
+  derive_constants(h);
+  HIP_TRY(hipMemcpy(h->d_params, h->params, sizeof(double) * GAL_NPARAM, hipMemcpyHostToDevice));
+  h->have_physics = true;
+  return GAL_OK;
+}
+
+int gal_ode_batch_set_cooling_table(gal_ode_batch* h, const double* ln_lambda, int n_t, double ln_t_min,
+                                    double d_ln_t, int n_z, double ln_z_min, double d_ln_z) {
+  if (!h || !ln_lambda || n_t < 2 || n_z < 2 || n_t > GAL_COOLING_NT_MAX || n_z > GAL_COOLING_NZ_MAX ||
+      !(d_ln_t > 0.0) || !(d_ln_z > 0.0))
+    return GAL_E_INVALID;
+  CoolingDesc& c = h->cd;
+  c.n_t = n_t;
+  c.n_z = n_z;
+  c.ln_t_min = ln_t_min;
+  c.d_ln_t = d_ln_t;
+  c.inv_d_ln_t = 1.0 / d_ln_t;
+  c.ln_z_min = ln_z_min;
+  c.d_ln_z = d_ln_z;
+  c.inv_d_ln_z = 1.0 / d_ln_z;
+  c.t_min = std::exp(ln_t_min);
+  c.t_max = std::exp(ln_t_min + d_ln_t * (n_t - 1));
+  c.z_min = std::exp(ln_z_min);
+  c.z_max = std::exp(ln_z_min + d_ln_z * (n_z - 1));
+  HIP_TRY(hipMemcpy(h->d_table, ln_lambda, sizeof(double) * n_t * n_z, hipMemcpyHostToDevice));
+  h->have_table = true;
+  return GAL_OK;
+}
+
+int gal_ode_batch_set_launch(gal_ode_batch* h, int nodes_per_block, int blocks_per_cu) {
+  if (!h) return GAL_E_INVALID;
+  if (nodes_per_block != 64) return GAL_E_INVALID;
+  // the stage derivatives of all lanes of a CU must fit its 160 KiB of LDS
+  if (blocks_per_cu < 1 || (size_t)blocks_per_cu * sizeof(double) * kLdsVectors * kNV * nodes_per_block > 163840)
+    return GAL_E_INVALID;
+  h->nodes_per_block = nodes_per_block;
+  h->blocks_per_cu = blocks_per_cu;
+  return GAL_OK;
+}
+
+static int launch_evolve(gal_ode_batch* h, const EvolveArgs& args, hipStream_t s) {
+  const int blocks = std::max(1, h->n_cus * h->blocks_per_cu);
+  hipLaunchKernelGGL(reset_counter_kernel, dim3(1), dim3(64), 0, s, args.work);
+  hipLaunchKernelGGL(evolve_kernel<64>, dim3(blocks), dim3(64), lds_bytes<64>(), s, args);
+  HIP_TRY(hipGetLastError());
+  return GAL_OK;
+}
+
+static int make_args(gal_ode_batch* h, int n, int ld, double* y, const double* node_par, const double* t_start,
+                     const double* t_end, double* h_step, double* scale, int* status, int64_t* counters,
+                     EvolveArgs& a) {
+  if (!h || n < 0 || n > h->n_max || ld < n) return GAL_E_INVALID;
+  if (!h->have_physics || !h->have_table) return GAL_E_INVALID;
+  if (n > 0 && (!y || !node_par || !t_start || !t_end || !h_step || !status)) return GAL_E_INVALID;
+  if ((h->flags & GAL_FLAG_SCALES_PROVIDED) && !scale) return GAL_E_INVALID;
+  a.n = n;
+  a.ld = ld;
+  a.y = y;
+  a.np = node_par;
+  a.t_start = t_start;
+  a.t_end = t_end;
+  a.h_step = h_step;
+  a.scale = scale;
+  a.status = status;
+  a.counters = reinterpret_cast<long long*>(counters);
+  a.order = nullptr;
+  a.work = h->d_work;
+  a.P = h->d_params;
+  a.tab = h->d_table;
+  a.D = h->D;
+  a.cd = h->cd;
+  a.eps_abs = h->tol_abs;
+  a.eps_rel = h->tol_rel;
+  a.flags = h->flags;
+  a.mode = (int)h->params[GAL_P_PHYSICS_MODE];
+  return GAL_OK;
+}
+
+int gal_ode_batch_evolve_d(gal_ode_batch* h, int n, int ld, double* y, const double* node_par,
+                           const double* t_start, const double* t_end, double* h_step, double* scale, int* status,
+                           int64_t* counters, void* stream) {
+  EvolveArgs a;
+  int rc = make_args(h, n, ld, y, node_par, t_start, t_end, h_step, scale, status, counters, a);
+  if (rc != GAL_OK) return rc;
+  if (n == 0) return GAL_OK;
+  return launch_evolve(h, a, (hipStream_t)stream);
+}
+
+int gal_ode_batch_evolve_h(gal_ode_batch* h, int n, double* y, const double* node_par, const double* t_start,
+                           const double* t_end, double* h_step, double* scale, int* status, int64_t* counters) {
+  if (!h || n < 0 || n > h->n_max) return GAL_E_INVALID;
+  if (n == 0) return GAL_OK;
+  if (!y || !node_par || !t_start || !t_end || !h_step || !status) return GAL_E_INVALID;
+  if (!h->d_y) {
+    const size_t nm = (size_t)h->n_max;
+    if (hipMalloc(&h->d_y, sizeof(double) * GAL_NVAR * nm) != hipSuccess ||
+        hipMalloc(&h->d_np, sizeof(double) * GAL_NPARAM_NODE * nm) != hipSuccess ||
+        hipMalloc(&h->d_t0, sizeof(double) * nm) != hipSuccess ||
+        hipMalloc(&h->d_t1, sizeof(double) * nm) != hipSuccess ||
+        hipMalloc(&h->d_h, sizeof(double) * nm) != hipSuccess ||
+        hipMalloc(&h->d_scale, sizeof(double) * GAL_NVAR * nm) != hipSuccess ||
+        hipMalloc(&h->d_status, sizeof(int) * nm) != hipSuccess ||
+        hipMalloc(&h->d_counters, sizeof(long long) * 3 * nm) != hipSuccess) {
+      free_staging(h);
+      return GAL_E_NOMEM;
+    }
+  }
+  const size_t nn = (size_t)n;
+  HIP_TRY(hipMemcpy(h->d_y, y, sizeof(double) * GAL_NVAR * nn, hipMemcpyHostToDevice));
+  HIP_TRY(hipMemcpy(h->d_np, node_par, sizeof(double) * GAL_NPARAM_NODE * nn, hipMemcpyHostToDevice));
+  HIP_TRY(hipMemcpy(h->d_t0, t_start, sizeof(double) * nn, hipMemcpyHostToDevice));
+  HIP_TRY(hipMemcpy(h->d_t1, t_end, sizeof(double) * nn, hipMemcpyHostToDevice));
+  HIP_TRY(hipMemcpy(h->d_h, h_step, sizeof(double) * nn, hipMemcpyHostToDevice));
+  if ((h->flags & GAL_FLAG_SCALES_PROVIDED) && scale)
+    HIP_TRY(hipMemcpy(h->d_scale, scale, sizeof(double) * GAL_NVAR * nn, hipMemcpyHostToDevice));
+  int rc = gal_ode_batch_evolve_d(h, n, n, h->d_y, h->d_np, h->d_t0, h->d_t1, h->d_h, h->d_scale, h->d_status,
+                                  counters ? reinterpret_cast<int64_t*>(h->d_counters) : nullptr, nullptr);
+  if (rc != GAL_OK) return rc;
+  HIP_TRY(hipDeviceSynchronize());
+  HIP_TRY(hipMemcpy(y, h->d_y, sizeof(double) * GAL_NVAR * nn, hipMemcpyDeviceToHost));
+  HIP_TRY(hipMemcpy(h_step, h->d_h, sizeof(double) * nn, hipMemcpyDeviceToHost));
+  HIP_TRY(hipMemcpy(status, h->d_status, sizeof(int) * nn, hipMemcpyDeviceToHost));
+  if (scale) HIP_TRY(hipMemcpy(scale, h->d_scale, sizeof(double) * GAL_NVAR * nn, hipMemcpyDeviceToHost));
+  if (counters) HIP_TRY(hipMemcpy(counters, h->d_counters, sizeof(long long) * 3 * nn, hipMemcpyDeviceToHost));
+  return GAL_OK;
+}
+
+int gal_ode_rhs_d(gal_ode_batch* h, int n, int ld, const double* t, const double* y, const double* node_par,
+                  double* dydt, void* stream) {
+  if (!h || n < 0 || ld < n || !h->have_physics || !h->have_table) return GAL_E_INVALID;
+  if (n == 0) return GAL_OK;
+  if (!t || !y || !node_par || !dydt) return GAL_E_INVALID;
+  const int threads = 256;
+  hipLaunchKernelGGL(rhs_kernel, dim3((n + threads - 1) / threads), dim3(threads), 0, (hipStream_t)stream, n, ld, t,
+                     y, node_par, dydt, (const double*)h->d_params, (const double*)h->d_table, h->D, h->cd,
+                     (int)h->params[GAL_P_PHYSICS_MODE]);
+  HIP_TRY(hipGetLastError());
+  return GAL_OK;
+}
+
+int gal_ode_hadjust_d(int n, int dim, int ord, double eps_abs, double eps_rel, double a_y, double a_dydt,
+                      const double* y, const double* yerr, const double* yp, const double* scale,
+                      const int* is_non_negative, double* h, int* result, void* stream) {
+  if (n < 0 || dim < 1 || ord < 1) return GAL_E_INVALID;
+  if (n == 0) return GAL_OK;
+  if (!y || !yerr || !yp || !scale || !is_non_negative || !h || !result) return GAL_E_INVALID;
+  const int threads = 256;
+  hipLaunchKernelGGL(hadjust_kernel, dim3((n + threads - 1) / threads), dim3(threads), 0, (hipStream_t)stream, n, dim,
+                     ord, eps_abs, eps_rel, a_y, a_dydt, y, yerr, yp, scale, is_non_negative, h, result);
+  HIP_TRY(hipGetLastError());
+  return GAL_OK;
+}
+
+int gal_ode_batch_autotune(gal_ode_batch* h, int n, int ld, double* y, const double* node_par,
+                           const double* t_start, const double* t_end, double* h_step, double* scale, int* status,
+                           void* stream, int* best_nodes_per_block, int* best_blocks_per_cu) {
+  // Times each admissible launch shape on copies of the inputs (the caller's
+  // arrays are untouched), keeps the fastest and leaves it set on the handle.
+  if (!h || n <= 0 || ld < n) return GAL_E_INVALID;
+  hipStream_t s = (hipStream_t)stream;
+  double *wy = nullptr, *wh = nullptr, *wsc = nullptr;
+  int* wst = nullptr;
+  const size_t nn = (size_t)ld;
+  if (hipMalloc(&wy, sizeof(double) * GAL_NVAR * nn) != hipSuccess ||
+      hipMalloc(&wh, sizeof(double) * nn) != hipSuccess ||
+      hipMalloc(&wsc, sizeof(double) * GAL_NVAR * nn) != hipSuccess ||
+      hipMalloc(&wst, sizeof(int) * nn) != hipSuccess) {
+    hipFree(wy); hipFree(wh); hipFree(wsc); hipFree(wst);
+    return GAL_E_NOMEM;
+  }
+  const int shapes[][2] = {{64, 1}, {64, 2}};
+  float best = 1e30f;
+  int bn = h->nodes_per_block, bb = h->blocks_per_cu;
+  hipEvent_t e0, e1;
+  hipEventCreate(&e0);
+  hipEventCreate(&e1);
+  int rc = GAL_OK;
+  for (auto& sh : shapes) {
+    if (gal_ode_batch_set_launch(h, sh[0], sh[1]) != GAL_OK) continue;
+    for (int rep = 0; rep < 2 && rc == GAL_OK; ++rep) {
+      hipMemcpyAsync(wy, y, sizeof(double) * GAL_NVAR * nn, hipMemcpyDeviceToDevice, s);
+      hipMemcpyAsync(wh, h_step, sizeof(double) * nn, hipMemcpyDeviceToDevice, s);
+      if (scale && (h->flags & GAL_FLAG_SCALES_PROVIDED))
+        hipMemcpyAsync(wsc, scale, sizeof(double) * GAL_NVAR * nn, hipMemcpyDeviceToDevice, s);
+      hipEventRecord(e0, s);
+      rc = gal_ode_batch_evolve_d(h, n, ld, wy, node_par, t_start, t_end, wh, wsc, wst, nullptr, stream);
+      hipEventRecord(e1, s);
+      hipEventSynchronize(e1);
+      float ms = 0.0f;
+      hipEventElapsedTime(&ms, e0, e1);
+      if (rep == 1 && ms < best) {
+        best = ms;
+        bn = sh[0];
+        bb = sh[1];
+      }
+    }
+  }
+  hipEventDestroy(e0);
+  hipEventDestroy(e1);
+  hipFree(wy); hipFree(wh); hipFree(wsc); hipFree(wst);
+  (void)status;
+  gal_ode_batch_set_launch(h, bn, bb);
+  if (best_nodes_per_block) *best_nodes_per_block = bn;
+  if (best_blocks_per_cu) *best_blocks_per_cu = bb;
+  return rc;
+}
+
+}  // extern "C"

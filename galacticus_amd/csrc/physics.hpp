@@ -1,0 +1,594 @@
+// physics.hpp -- device RHS of the node ODE system (one node per lane).
+//
+// dy/dt = nodeOperatorMulti%differentialEvolution (nodes/operators/multi.F90:313-332)
+// over the operators of parameters/quickTest.xml:220-289, in list order,
+// after the analytic solutions of the basic component
+// (nodes/operators/physics/cosmic_time.F90:81-95,
+//  dark_matter_only_mass/interpolate.F90:215-235).  Each block cites the
+// reference routine whose rate it computes.
+//
+// The formulas are the reference's; the evaluation is arranged for a GPU
+// lane with a long dependent chain, where every instruction counts:
+//  * The cooling radius -- a bracketed numerical root of
+//    t_cool(r) = t_available in the reference (cooling_radius/simple.F90:365-381)
+//    -- is evaluated in closed form: with the isothermal (virial) temperature
+//    profile and a CIE cooling function, t_cool(r) = K / rho(r), and the
+//    beta=2/3 profile inverts to r = r_c sqrt(rho0 t_avail / K - 1).
+//  * Power laws whose exponent is a multiple of 1/2 (all four of
+//    quickTest.xml: -1.5, 2, 3.5, 3.5) are products of the base and its
+//    square root instead of exp(alpha ln x); other exponents use exp/log.
+//  * a^3 and the virial overdensity are formed without cbrt:
+//    Delta_m rho_mean = Delta_c rho_crit0 E^2(a), E^2 = Omega_m/a^3 + Omega_L.
+//  * The jet power's two power laws and spin exponential share one exp:
+//    (M/1e9)^0.9 (Mdot/Mdot_Edd)^1.2 = C M^-0.3 Mdot^1.2.
+//  * Constants that depend only on the parameters are computed on the host
+//    (DerivedConstants).
+#pragma once
+#include <hip/hip_runtime.h>
+#include "../../include/gal_ode.h"
+#include "constants.hpp"
+
+namespace gal {
+
+constexpr int kNV = GAL_NVAR;
+constexpr int kNoHalfPower = -100000;
+
+// Constants derived from the physics parameters once per set_physics call.
+struct DerivedConstants {
+  double om, ol;          // Omega_m, Omega_L
+  double a3_norm;         // Omega_m/Omega_L: a^3 = a3_norm sinh^2(sinh_arg t)
+  double sinh_arg;        // 1.5 sqrt(Omega_L) H0 [1/Gyr]
+  double rvir_coeff;      // 3 / (4 pi rho_crit0)
+  double a3_reion;        // (1/(1+z_reion))^3
+  double f_baryon;        // Omega_b / Omega_m
+  double inv_sqrt_200;    // 1/sqrt(200): dynamical-time velocity zero point
+  double inv_200;
+  double v_fb_disk, sqrt_v_fb_disk, v_fb_sph, sqrt_v_fb_sph;
+  double ln_200, ln_v_fb_disk, ln_v_fb_sph;
+  int n2_sf_disk, n2_sf_sph, n2_fb_disk, n2_fb_sph;  // 2*exponent, or kNoHalfPower
+  double t_vir_per_v2;    // 0.5 mu m_u (1e3)^2 / k
+  double cs2_per_t;       // (5/3) k / (mu m_u) / (1e3)^2  [(km/s)^2 / K]
+  double cs2_sph;         // cs^2 at bondiHoyleAccretionTemperatureSpheroid
+  double m_edd_per_mass;  // Eddington accretion rate per Msun [1/Gyr]
+  double ln_jet_kerr;     // ln of the jet-power prefactor for spin > 0.8
+  double ln_jet_schw;     // ln of the jet-power prefactor for spin <= 0.8
+  double cool_k_pre;      // (dof/2) k / ergs / Gyr
+  double unit_energy;     // Msun (km/s)^2 / Gyr -> erg/s
+  double nh_coeff;        // Msun / m_H / (Mpc in cm)^3
+};
+
+// Cooling table descriptor (table itself in global memory).
+struct CoolingDesc {
+  int n_t, n_z;
+  double ln_t_min, d_ln_t, inv_d_ln_t;
+  double ln_z_min, d_ln_z, inv_d_ln_z;
+  double t_min, t_max, z_min, z_max;
+};
+
+struct NodePar {
+  double t_ref, m_ref, m_rate, spin_rate;
+  bool satellite;
+};
+
+__device__ __forceinline__ double sq(double x) { return x * x; }
+
+// x^(n2/2) for integer n2 given sqrt(x); wave-uniform n2.
+__device__ __forceinline__ double pow_half(double x, double sqrt_x, int n2) {
+  const int m = n2 < 0 ? -n2 : n2;
+  double r = (m & 1) ? sqrt_x : 1.0;
+  double b = x;
+  for (int k = m >> 1; k; k >>= 1) {
+    if (k & 1) r *= b;
+    b *= b;
+  }
+  return n2 < 0 ? 1.0 / r : r;
+}
+
+// x^alpha: half-integer fast path, else exp(alpha ln x).
+__device__ __forceinline__ double pow_param(double x, double sqrt_x, double ln_x_fn_arg, int n2, double alpha) {
+  if (n2 != kNoHalfPower) return pow_half(x, sqrt_x, n2);
+  return exp(alpha * ln_x_fn_arg);
+}
+
+// Bilinear interpolation of ln(Lambda) in (ln T, ln Z) with "fix"
+// extrapolation: cieFileInterpolatingFactors / cieFileInterpolate
+// (cooling/cooling_function/CIE_file.F90:665-735).  Grids are uniform, so
+// the bracketing index is computed directly instead of by bisection.
+__device__ __forceinline__ double cooling_function(const double* __restrict__ tab, const CoolingDesc& c,
+                                                   double temperature, double z_solar) {
+  const double T = fmin(fmax(temperature, c.t_min), c.t_max);
+  const double Z = fmin(fmax(z_solar, c.z_min), c.z_max);
+  const double lt = log(T), lz = log(Z);
+  int it = (int)((lt - c.ln_t_min) * c.inv_d_ln_t);
+  int iz = (int)((lz - c.ln_z_min) * c.inv_d_ln_z);
+  it = it < 0 ? 0 : (it > c.n_t - 2 ? c.n_t - 2 : it);
+  iz = iz < 0 ? 0 : (iz > c.n_z - 2 ? c.n_z - 2 : iz);
+  const double t0 = c.ln_t_min + c.d_ln_t * it, t1 = c.ln_t_min + c.d_ln_t * (it + 1);
+  const double z0 = c.ln_z_min + c.d_ln_z * iz, z1 = c.ln_z_min + c.d_ln_z * (iz + 1);
+  const double ht = (lt - t0) / (t1 - t0);
+  const double hz = (lz - z0) / (z1 - z0);
+  const double* r0 = tab + iz * c.n_t + it;
+  const double* r1 = r0 + c.n_t;
+  const double v = r0[0] * (1.0 - ht) * (1.0 - hz) + r1[0] * (1.0 - ht) * hz + r0[1] * ht * (1.0 - hz) +
+                   r1[1] * ht * hz;
+  return exp(v);
+}
+
+// ISCO specific energy and angular momentum, prograde, gravitational units
+// (black_holes/fundamentals.F90; Bardeen, Press & Teukolsky 1972).
+__device__ __forceinline__ void isco(double a, double& energy, double& ang_mom) {
+  const double cp = cbrt(1.0 + a), cm = cbrt(1.0 - a);
+  const double a1 = 1.0 + cp * cm * (cp + cm);
+  const double a2 = sqrt(3.0 * a * a + a1 * a1);
+  const double r = 3.0 + a2 - sqrt((3.0 - a1) * (3.0 + a1 + 2.0 * a2));
+  const double sr = sqrt(r);
+  const double inv_den = 1.0 / (r * sqrt(r * r - 3.0 * r + 2.0 * a * sr));
+  energy = (r * r - 2.0 * r + a * sr) * inv_den;
+  ang_mom = (r * r - 2.0 * a * sr + a * a) * sr * inv_den;
+}
+
+__device__ __forceinline__ double hernquist_density(double mass, double scale, double r) {
+  if (!(mass > 0.0) || !(scale > 0.0) || !(r > 0.0)) return 0.0;
+  const double rs = r + scale;
+  return mass * scale / (2.0 * kPi * r * rs * rs * rs);
+}
+
+// Halo quantities shared by the RHS, the scale rules and the post-step.
+struct Halo {
+  double mass, a3, r_vir, v_vir, inv_v, t_dyn, lambda, r_gal;
+};
+
+__device__ __forceinline__ Halo halo(const double* __restrict__ P, const DerivedConstants& D, const NodePar& np,
+                                     double t, double spin_j) {
+  Halo h;
+  const double mrate = np.satellite ? 0.0 : np.m_rate;
+  h.mass = np.m_ref + mrate * (t - np.t_ref);
+  const double s = sinh(D.sinh_arg * t);
+  h.a3 = D.a3_norm * s * s;
+  const double om_a3 = D.om / h.a3;
+  const double e2 = om_a3 + D.ol;
+  const double om_a = om_a3 / e2;
+  const double x = om_a - 1.0;
+  const double delta_c = 18.0 * kPi * kPi + 82.0 * x - 39.0 * x * x;
+  h.r_vir = cbrt(D.rvir_coeff * h.mass / (delta_c * e2));
+  const double u = kGravity * h.mass / h.r_vir;
+  h.inv_v = rsqrt(u);
+  h.v_vir = u * h.inv_v;
+  h.t_dyn = kMpcPerKmsToGyr * h.r_vir * h.inv_v;
+  const double jscale = h.mass * h.v_vir * h.r_vir;
+  h.lambda = jscale > 0.0 ? spin_j / jscale : 0.0;
+  h.r_gal = P[GAL_P_RADIUS_FACTOR] * h.r_vir * h.lambda;  // fixed.F90:318-325
+  return h;
+}
+
+// ---- the RHS -------------------------------------------------------------
+__device__ __forceinline__ void rhs(const double* __restrict__ P, const DerivedConstants& D,
+                                    const double* __restrict__ tab, const CoolingDesc& cd, int mode,
+                                    const NodePar& np, double t, const double (&y)[kNV], double (&f)[kNV]) {
+#pragma unroll
+  for (int i = 0; i < kNV; ++i) f[i] = 0.0;
+  if (mode == GAL_PHYSICS_TEST) {
+    // source/tests/ODE_solver/functions.F90:37-80
+    f[0] = sin(t);
+    f[1] = y[2];
+    f[2] = -1.0 * y[1];
+    return;
+  }
+  const Halo H = halo(P, D, np, t, y[GAL_V_SPIN_ANGULAR_MOMENTUM]);
+  const double v_gal = H.v_vir, r_gal = H.r_gal;
+  const double sqrt_v = sqrt(v_gal);
+  const double inv_sqrt_v = 1.0 / sqrt_v;
+
+  // spin%angularMomentum growth (halo_angular_momentum_interpolate.F90)
+  f[GAL_V_SPIN_ANGULAR_MOMENTUM] += np.satellite ? 0.0 : np.spin_rate;
+
+  const double R = P[GAL_P_RECYCLED_FRACTION], yld = P[GAL_P_METAL_YIELD];
+  const double tdyn_gal = kMpcPerKmsToGyr * r_gal * H.inv_v;
+  const bool generic_pow = D.n2_sf_disk == kNoHalfPower || D.n2_sf_sph == kNoHalfPower ||
+                           D.n2_fb_disk == kNoHalfPower || D.n2_fb_sph == kNoHalfPower;
+  const double ln_v = generic_pow ? log(v_gal) : 0.0;
+
+  // ---- starFormationDisks (star_formation/disks.F90:200-325) -----------
+  const double mg_d = y[GAL_V_DISK_MASS_GAS], ms_d = y[GAL_V_DISK_MASS_STELLAR];
+  const double j_d = y[GAL_V_DISK_ANGULAR_MOMENTUM], zg_d = y[GAL_V_DISK_ABUNDANCES_GAS];
+  const bool disk_ok = !(j_d < 0.0 || r_gal < 0.0 || mg_d < 0.0);
+  const double inv_mg_d = mg_d > 0.0 ? 1.0 / mg_d : 0.0;
+  double sfr_d = 0.0;
+  if (disk_ok) {
+    double tau;
+    if (mode == GAL_PHYSICS_BOX) {
+      tau = P[GAL_P_BOX_TIMESCALE];  // starFormationTimescale fixed
+    } else {
+      // dynamicalTimeTimescale (star_formation/timescales/dynamical_time.F90:164-188)
+      const double eff = P[GAL_P_SF_DISK_EFFICIENCY];
+      const double pw = pow_param(v_gal * D.inv_200, sqrt_v * D.inv_sqrt_200, ln_v - D.ln_200, D.n2_sf_disk,
+                                  P[GAL_P_SF_DISK_EXPONENT_VELOCITY]);
+      tau = (v_gal <= 0.0 || eff == 0.0) ? 0.0 : fmax(tdyn_gal * pw / eff, P[GAL_P_SF_DISK_TIMESCALE_MINIMUM]);
+    }
+    sfr_d = tau > 0.0 ? mg_d / tau : 0.0;  // star_formation/rates/disks/timescale.F90:109-130
+    // instantaneousRates (stellar_populations/properties/instantaneous.F90:133-182)
+    const double zfuel = zg_d * inv_mg_d;
+    const double rs = (1.0 - R) * sfr_d;
+    f[GAL_V_DISK_MASS_STELLAR] += rs;
+    f[GAL_V_DISK_MASS_GAS] += -rs;
+    f[GAL_V_DISK_ABUNDANCES_STELLAR] += zfuel * rs;
+    f[GAL_V_DISK_ABUNDANCES_GAS] += -(zfuel * rs) + yld * sfr_d;
+    f[GAL_V_DISK_MASS_STELLAR_FORMED] += sfr_d;
+  }
+  // ---- starFormationSpheroids (star_formation/spheroids.F90:152) --------
+  const double mg_s = y[GAL_V_SPHEROID_MASS_GAS], ms_s = y[GAL_V_SPHEROID_MASS_STELLAR];
+  const double j_s = y[GAL_V_SPHEROID_ANGULAR_MOMENTUM], zg_s = y[GAL_V_SPHEROID_ABUNDANCES_GAS];
+  const bool sph_ok = mode == GAL_PHYSICS_STANDARD && !(j_s < 0.0 || r_gal < 0.0 || mg_s < 0.0);
+  const double inv_mg_s = mg_s > 0.0 ? 1.0 / mg_s : 0.0;
+  double sfr_s = 0.0;
+  if (sph_ok) {
+    const double eff = P[GAL_P_SF_SPHEROID_EFFICIENCY];
+    const double pw = pow_param(v_gal * D.inv_200, sqrt_v * D.inv_sqrt_200, ln_v - D.ln_200, D.n2_sf_sph,
+                                P[GAL_P_SF_SPHEROID_EXPONENT_VELOCITY]);
+    const double tau =
+        (v_gal <= 0.0 || eff == 0.0) ? 0.0 : fmax(tdyn_gal * pw / eff, P[GAL_P_SF_SPHEROID_TIMESCALE_MINIMUM]);
+    sfr_s = tau > 0.0 ? mg_s / tau : 0.0;
+    const double zfuel = zg_s * inv_mg_s;
+    const double rs = (1.0 - R) * sfr_s;
+    f[GAL_V_SPHEROID_MASS_STELLAR] += rs;
+    f[GAL_V_SPHEROID_MASS_GAS] += -rs;
+    f[GAL_V_SPHEROID_ABUNDANCES_STELLAR] += zfuel * rs;
+    f[GAL_V_SPHEROID_ABUNDANCES_GAS] += -(zfuel * rs) + yld * sfr_s;
+    f[GAL_V_SPHEROID_MASS_STELLAR_FORMED] += sfr_s;
+  }
+  // ---- stellarFeedbackDisks (stellar_feedback/disks.F90:116-206) --------
+  // powerLaw (outflows/power_law/_class.F90:118-164) limited by rateLimit
+  // (outflows/rate_limit.F90:111-170); outflowing gas joins the outflowed
+  // reservoir (hot_halo/standard/_class.F90:598-626, fractionStripped=0).
+  const double tdyn_fb = (v_gal <= 0.0 || r_gal <= 0.0) ? 1.0 : tdyn_gal;
+  if (disk_ok) {
+    double out;
+    if (mode == GAL_PHYSICS_BOX) {
+      out = P[GAL_P_BOX_OUTFLOW_FRACTION] * sfr_d;
+    } else {
+      const double w = D.v_fb_disk * H.inv_v;
+      out = v_gal <= 0.0 ? 0.0
+                         : pow_param(w, D.sqrt_v_fb_disk * inv_sqrt_v, D.ln_v_fb_disk - ln_v, D.n2_fb_disk,
+                                     P[GAL_P_FB_DISK_EXPONENT]) * sfr_d;
+    }
+    const double omax = fmax(mg_d / tdyn_fb / P[GAL_P_FB_DISK_TIMESCALE_MINIMUM], 0.0);
+    if (out > omax) out = out * omax / out;
+    if (out > 0.0) {
+      const double mtot = mg_d + ms_d;
+      const double rj = mtot > 0.0 ? j_d * (out / mtot) : 0.0;
+      const double rz = mg_d > 0.0 ? (zg_d * inv_mg_d) * out : 0.0;
+      f[GAL_V_HOT_OUTFLOWED_MASS] += out;
+      f[GAL_V_DISK_MASS_GAS] -= out;
+      f[GAL_V_HOT_OUTFLOWED_ANGULAR_MOMENTUM] += rj;
+      f[GAL_V_DISK_ANGULAR_MOMENTUM] -= rj;
+      f[GAL_V_HOT_OUTFLOWED_ABUNDANCES] += rz;
+      f[GAL_V_DISK_ABUNDANCES_GAS] -= rz;
+    }
+  }
+  if (mode == GAL_PHYSICS_BOX) return;
+  // ---- stellarFeedbackSpheroids (stellar_feedback/spheroids.F90:116) ----
+  if (sph_ok) {
+    const double w = D.v_fb_sph * H.inv_v;
+    double out = v_gal <= 0.0 ? 0.0
+                              : pow_param(w, D.sqrt_v_fb_sph * inv_sqrt_v, D.ln_v_fb_sph - ln_v, D.n2_fb_sph,
+                                          P[GAL_P_FB_SPHEROID_EXPONENT]) * sfr_s;
+    const double omax = fmax(mg_s / tdyn_fb / P[GAL_P_FB_SPHEROID_TIMESCALE_MINIMUM], 0.0);
+    if (out > omax) out = out * omax / out;
+    if (out > 0.0) {
+      const double mtot = mg_s + ms_s;
+      const double rj = mtot > 0.0 ? j_s * (out / mtot) : 0.0;
+      const double rz = mg_s > 0.0 ? (zg_s * inv_mg_s) * out : 0.0;
+      f[GAL_V_HOT_OUTFLOWED_MASS] += out;
+      f[GAL_V_SPHEROID_MASS_GAS] -= out;
+      f[GAL_V_HOT_OUTFLOWED_ANGULAR_MOMENTUM] += rj;
+      f[GAL_V_SPHEROID_ANGULAR_MOMENTUM] -= rj;
+      f[GAL_V_HOT_OUTFLOWED_ABUNDANCES] += rz;
+      f[GAL_V_SPHEROID_ABUNDANCES_GAS] -= rz;
+    }
+  }
+  // ---- hot-halo structure and cooling radius ----------------------------
+  // beta=2/3 profile, core coreRadiusOverVirialRadius*r_vir, truncated at
+  // outerRadius (hot_halo/mass_distribution/beta_profile.F90); virial
+  // (isothermal) temperature (hot_halo/temperature_profile/virial.F90).
+  const double m_hot = y[GAL_V_HOT_MASS], r_out = y[GAL_V_HOT_OUTER_RADIUS];
+  const double t_vir = D.t_vir_per_v2 * H.v_vir * H.v_vir;
+  const bool hot_ok = m_hot > 0.0 && r_out > 0.0;
+  double r_core = 1.0, rho0 = 0.0, r_cool = 0.0, r_cool_rate = 0.0, r_mean = 1.0, lambda_cool = 0.0;
+  double inv_m_hot = 0.0, nh_per_rho = 0.0, nall_per_nh = 0.0;
+  if (hot_ok) {
+    inv_m_hot = 1.0 / m_hot;
+    r_core = P[GAL_P_CORE_RADIUS_OVER_VIRIAL] * H.r_vir;
+    const double xo = r_out / r_core;
+    const double inv_shape = 1.0 / (xo - atan(xo));
+    rho0 = m_hot * inv_shape / (4.0 * kPi * r_core * r_core * r_core);
+    r_mean = r_core * (0.5 * xo * xo - 0.5 * log1p(xo * xo)) * inv_shape;
+    // composition of metallicity-only abundances, fully ionized H and He
+    const double zf = fmin(fmax(y[GAL_V_HOT_ABUNDANCES] * inv_m_hot, 0.0), 1.0);
+    constexpr double kNorm = 1.0 / (kHydrogenPrimordial + kHeliumPrimordial);
+    const double X = (1.0 - zf) * kHydrogenPrimordial * kNorm;
+    const double Y = (1.0 - zf) * kHeliumPrimordial * kNorm;
+    const double nh = X / kAtomicMassHydrogen, nhe = Y / kAtomicMassHelium;
+    nh_per_rho = X * D.nh_coeff;
+    nall_per_nh = (nh + nhe) / nh + (1.0 + 2.0 * nhe / nh);
+    lambda_cool = cooling_function(tab, cd, t_vir, zf * (1.0 / kMetallicitySolar));
+    // simpleTime (cooling_time/simple.F90:128-181): t_cool = K / rho
+    const double t_avail = H.t_dyn;  // White-Frenk.F90:144-146 (ageFactor = 0)
+    if (!(lambda_cool > 0.0)) {
+      r_cool = 0.0;  // t_cool = 1e10 Gyr everywhere
+    } else {
+      const double K = D.cool_k_pre * t_vir * nall_per_nh / (lambda_cool * nh_per_rho);
+      // simpleRadius (cooling_radius/simple.F90:313-387)
+      const double rho_out = rho0 / (1.0 + xo * xo);
+      if (K / rho_out - t_avail < 0.0) r_cool = r_out;
+      else if (K / rho0 - t_avail > 0.0) r_cool = 0.0;
+      else r_cool = fmin(r_core * sqrt(fmax(rho0 * t_avail / K - 1.0, 0.0)), r_out);
+    }
+    // simpleRadiusGrowthRate (cooling_radius/simple.F90:229-309); the
+    // time-available growth rate is 1 (White-Frenk.F90:161-168)
+    if (r_cool < r_out && r_cool > 0.0) {
+      const double x = r_cool / r_core;
+      const double slope = 2.0 * x * x / (1.0 + x * x);
+      r_cool_rate = slope != 0.0 ? r_cool / t_avail / slope : 0.0;
+    }
+  }
+  // ---- blackHolesAccretion (black_holes/accretion.F90:113-170) ----------
+  // rates: accretion_rates/standard.F90:236-440; Bondi-Hoyle-Lyttleton:
+  // accretion/Bondi_Hoyle_Lyttleton.F90:34-78; thin disk:
+  // accretion_disks/Shakura_Sunyaev.F90:70-179.
+  double p_jet = 0.0;
+  const double m_bh = y[GAL_V_BH_MASS];
+  if (m_bh > 0.0) {
+    const double spin = fmin(fmax(y[GAL_V_BH_SPIN], 0.0), 0.998);
+    double e_isco, l_isco;
+    isco(spin, e_isco, l_isco);
+    const double eff_thin = 1.0 - e_isco;
+    const double m_edd = D.m_edd_per_mass * m_bh;
+    const double edd_cap = eff_thin > 0.0 ? m_edd / eff_thin : 0.0;
+    constexpr double kBhlUnit = kKilo * kGigaYear / kMegaParsec * 4.0 * kPi;
+    // spheroid mode
+    double mdot_sph = 0.0;
+    {
+      double racc = P[GAL_P_BH_TEMPERATURE_SPHEROID] > 0.0 ? kGravity * m_bh / D.cs2_sph : HUGE_VAL;
+      racc = fmax(racc, y[GAL_V_BH_RADIAL_POSITION]);
+      double rho = hernquist_density(mg_s, r_gal, racc);
+      if (rho > 1.0) {
+        const double lj = fmin(sqrt(D.cs2_sph / (kGravity * rho)), r_gal);
+        if (lj > racc) rho = hernquist_density(mg_s, r_gal, lj);
+        double m = fmax(P[GAL_P_BH_ENHANCEMENT_SPHEROID] * kBhlUnit * sq(kGravity * m_bh) * rho /
+                            (D.cs2_sph * sqrt(D.cs2_sph)),
+                        0.0);
+        if (m > 0.0 && eff_thin > 0.0) m = fmin(m, edd_cap);
+        mdot_sph = m;
+      }
+    }
+    // hot-halo mode (hot mode only)
+    double mdot_hot = 0.0;
+    if (hot_ok) {
+      const double cs2 = D.cs2_per_t * t_vir;
+      const double r2 = fmin(kGravity * m_bh / cs2, r_out);
+      const double q = r_cool / H.r_vir;
+      double fhot;  // standardHotModeFraction (accretion_rates/standard.F90:442-466)
+      if (q < 0.9) fhot = 1.0;
+      else if (q > 1.0) fhot = 0.0;
+      else { const double x = (q - 0.9) / (1.0 - 0.9); fhot = x * x * (2.0 * x - 3.0) + 1.0; }
+      const double rho2 = fhot * (rho0 / (1.0 + sq(r2 / r_core)));
+      if (rho2 > 1.0) {
+        const double cs = sqrt(cs2);
+        double m = fmax(P[GAL_P_BH_ENHANCEMENT_HOT_HALO] * kBhlUnit * r2 * r2 * rho2 * cs, 0.0);
+        const double mmax = fmax(m_hot * cs / (r_out * kMpcPerKmsToGyr), 0.0);
+        m = fmin(m, mmax);
+        if (m > 0.0 && eff_thin > 0.0) m = fmin(m, edd_cap);
+        mdot_hot = m;
+      }
+    }
+    const double mdot = mdot_sph + mdot_hot;
+    if (mdot > 0.0) {
+      // Meier (2001) jet power (Shakura_Sunyaev.F90:92-140)
+      const double lm = -0.3 * log(m_bh) + 1.2 * log(mdot);
+      p_jet = spin > 0.8 ? exp(D.ln_jet_kerr + lm) * (1.0 + 1.1 * spin + 0.29 * spin * spin)
+                         : exp(D.ln_jet_schw + lm + 3.785 * spin);
+      constexpr double kC2 = (kSpeedLight / kKilo) * (kSpeedLight / kKilo);
+      const double eff_jet = p_jet / mdot / kC2;
+      const double reduced = mdot * (1.0 - eff_thin - eff_jet);
+      const double spin_up = (l_isco - 2.0 * spin * e_isco) * mdot / m_bh;
+      f[GAL_V_BH_MASS] += reduced;
+      if (mdot_sph != 0.0) {
+        f[GAL_V_SPHEROID_MASS_GAS] -= mdot_sph;
+        if (mg_s > 0.0) f[GAL_V_SPHEROID_ABUNDANCES_GAS] -= mdot_sph * (zg_s * inv_mg_s);
+        if (mg_s + ms_s > 0.0) f[GAL_V_SPHEROID_ANGULAR_MOMENTUM] -= mdot_sph * (j_s / (mg_s + ms_s));
+      }
+      if (mdot_hot != 0.0 && m_hot > 0.0) {
+        f[GAL_V_HOT_MASS] -= mdot_hot;
+        f[GAL_V_HOT_ABUNDANCES] -= mdot_hot * (y[GAL_V_HOT_ABUNDANCES] * inv_m_hot);
+        f[GAL_V_HOT_ANGULAR_MOMENTUM] -= mdot_hot * (y[GAL_V_HOT_ANGULAR_MOMENTUM] * inv_m_hot);
+      }
+      f[GAL_V_BH_SPIN] += spin_up;
+    }
+  }
+  // ---- CGMAccretion (circumgalactic_medium/accretion.F90:517) with
+  //      accretionHalo simple (accretion/halo/simple.F90:281-378) ---------
+  if (!np.satellite && H.mass > 0.0) {
+    const double failed = (H.a3 > D.a3_reion && H.v_vir < P[GAL_P_VELOCITY_SUPPRESSION_REIONIZATION]) ? 1.0 : 0.0;
+    const double growth = np.m_rate / H.mass;
+    const double m_un = y[GAL_V_HOT_UNACCRETED_MASS], z_un = y[GAL_V_HOT_UNACCRETED_ABUNDANCES];
+    const double rate_acc = D.f_baryon * np.m_rate * (1.0 - failed) + m_un * growth * (1.0 - failed);
+    const double rate_fail = D.f_baryon * np.m_rate * failed - m_un * growth * (1.0 - failed);
+    const double rate_z = z_un * growth * (1.0 - failed);
+    f[GAL_V_HOT_MASS] += rate_acc;
+    f[GAL_V_HOT_UNACCRETED_MASS] += rate_fail;
+    f[GAL_V_HOT_ABUNDANCES] += rate_z;
+    f[GAL_V_HOT_UNACCRETED_ABUNDANCES] -= rate_z;
+    f[GAL_V_HOT_ANGULAR_MOMENTUM] += rate_acc * (y[GAL_V_SPIN_ANGULAR_MOMENTUM] / H.mass);
+  }
+  // ---- CGMOutflowReincorporation (outflow_reincorporation.F90:272-330)
+  //      haloDynamicalTimeRate (halo_dynamical_time.F90:113-125) ----------
+  {
+    const double m_o = y[GAL_V_HOT_OUTFLOWED_MASS];
+    if (m_o > 0.0) {
+      const double r = m_o * P[GAL_P_REINCORPORATION_MULTIPLIER] / H.t_dyn;
+      const double q = r / m_o;
+      const double rj = y[GAL_V_HOT_OUTFLOWED_ANGULAR_MOMENTUM] * q;
+      const double rz = y[GAL_V_HOT_OUTFLOWED_ABUNDANCES] * q;
+      f[GAL_V_HOT_OUTFLOWED_MASS] -= r;
+      f[GAL_V_HOT_OUTFLOWED_ANGULAR_MOMENTUM] -= rj;
+      f[GAL_V_HOT_OUTFLOWED_ABUNDANCES] -= rz;
+      f[GAL_V_HOT_MASS] += r;
+      f[GAL_V_HOT_ANGULAR_MOMENTUM] += rj;
+      f[GAL_V_HOT_ABUNDANCES] += rz;
+    }
+  }
+  // ---- CGMCoolingHeating (circumgalactic_medium/cooling_heating.F90:216-381)
+  //      whiteFrenk1991Rate (cooling_rate/White-Frenk.F90:132-185) ---------
+  if (hot_ok && y[GAL_V_HOT_ANGULAR_MOMENTUM] > 0.0) {
+    double cool;
+    if (H.v_vir > P[GAL_P_COOLING_VELOCITY_CUTOFF]) cool = 0.0;
+    else if (r_cool >= r_out) cool = m_hot / H.t_dyn;
+    else cool = 4.0 * kPi * r_cool * r_cool * (rho0 / (1.0 + sq(r_cool / r_core))) * r_cool_rate;
+    const double heat = P[GAL_P_EFFICIENCY_RADIO_MODE] * p_jet * sq(H.inv_v);
+    if (heat > cool) {
+      const double out = fmin(heat - cool, P[GAL_P_RATE_MAXIMUM_EXPULSION] * m_hot / H.t_dyn);
+      const double q = out * inv_m_hot;
+      const double rz = y[GAL_V_HOT_ABUNDANCES] * q;
+      const double rj = y[GAL_V_HOT_ANGULAR_MOMENTUM] * q;
+      f[GAL_V_HOT_MASS] -= out;
+      f[GAL_V_HOT_ABUNDANCES] -= rz;
+      f[GAL_V_HOT_ANGULAR_MOMENTUM] -= rj;
+      // hotHaloMassEjection: the ejected gas joins the outflowed reservoir
+      f[GAL_V_HOT_OUTFLOWED_MASS] += out;
+      f[GAL_V_HOT_OUTFLOWED_ABUNDANCES] += rz;
+      f[GAL_V_HOT_OUTFLOWED_ANGULAR_MOMENTUM] += rj;
+    } else if (cool > heat) {
+      cool = fmax(cool - heat, 0.0);
+      // constantRotation specific angular momentum: j(r) = (J/M) r / <r>
+      const double rj = cool * ((y[GAL_V_HOT_ANGULAR_MOMENTUM] * inv_m_hot) * r_cool / r_mean);
+      const double rz = cool * y[GAL_V_HOT_ABUNDANCES] * inv_m_hot;
+      f[GAL_V_HOT_MASS] -= cool;
+      f[GAL_V_HOT_ANGULAR_MOMENTUM] -= rj;
+      f[GAL_V_HOT_ABUNDANCES] -= rz;
+      f[GAL_V_DISK_MASS_GAS] += cool;
+      f[GAL_V_DISK_ABUNDANCES_GAS] += rz;
+      f[GAL_V_DISK_ANGULAR_MOMENTUM] += rj * (1.0 - P[GAL_P_FRACTION_LOSS_ANGULAR_MOMENTUM]);
+    }
+  }
+  // ---- coolingEnergyRadiated (cooling_energy_radiated.F90:200-260) -------
+  if (hot_ok) {
+    const double rho = m_hot / (4.0 / 3.0 * kPi * r_out * r_out * r_out);
+    const double nh = rho * nh_per_rho;
+    const double rcm = r_out * (kMegaParsec / kCenti);
+    const double count = nh * nall_per_nh * (4.0 * kPi / 3.0) * rcm * rcm * rcm;
+    f[GAL_V_HOT_ENERGY_RADIATED] += lambda_cool * nh * count;
+  }
+}
+
+// Component scale-set rules at the start of an evolve (hot_halo/standard/_class.F90:804-840,
+// disk/standard/_class.F90:735-828, spheroid/standard/_class.F90:804-860,
+// black_hole/standard/_class.F90:195-240, cooling_energy_radiated.F90:150-185).
+__device__ __forceinline__ void scales(const double* __restrict__ P, const DerivedConstants& D, const NodePar& np,
+                                       double t, const double (&y)[kNV], double (&sc)[kNV]) {
+  const Halo H = halo(P, D, np, t, y[GAL_V_SPIN_ANGULAR_MOMENTUM]);
+  const double mass = H.mass, r_vir = H.r_vir, v_vir = H.v_vir;
+  constexpr double mrel = 1.0e-3, rrel = 1.0e-1;
+#pragma unroll
+  for (int i = 0; i < kNV; ++i) sc[i] = 1.0;
+  sc[GAL_V_HOT_MASS] = mass * mrel;
+  sc[GAL_V_HOT_OUTFLOWED_MASS] = mass * mrel;
+  sc[GAL_V_HOT_UNACCRETED_MASS] = mass * mrel;
+  sc[GAL_V_HOT_ABUNDANCES] = mass * mrel;
+  sc[GAL_V_HOT_UNACCRETED_ABUNDANCES] = mass * mrel;
+  sc[GAL_V_HOT_OUTFLOWED_ABUNDANCES] = mass * mrel;
+  sc[GAL_V_HOT_ANGULAR_MOMENTUM] = mass * r_vir * v_vir * mrel;
+  sc[GAL_V_HOT_OUTFLOWED_ANGULAR_MOMENTUM] = mass * r_vir * v_vir * mrel;
+  sc[GAL_V_HOT_OUTER_RADIUS] = r_vir * rrel;
+  sc[GAL_V_HOT_STRIPPED_MASS] = mass * mrel;
+  sc[GAL_V_HOT_STRIPPED_ABUNDANCES] = mass * mrel;
+  sc[GAL_V_HOT_ENERGY_RADIATED] = D.unit_energy * mass * v_vir * v_vir * 1.0e-3;
+  const double ang = fmax(fabs(y[GAL_V_DISK_ANGULAR_MOMENTUM]) + fabs(y[GAL_V_SPHEROID_ANGULAR_MOMENTUM]), 0.1);
+  const double mcomp = fmax(fabs(y[GAL_V_DISK_MASS_GAS]) + fabs(y[GAL_V_SPHEROID_MASS_GAS]) +
+                                fabs(y[GAL_V_DISK_MASS_STELLAR]) + fabs(y[GAL_V_SPHEROID_MASS_STELLAR]),
+                            1.0);
+  const double zmin = fmax(mcomp * P[GAL_P_TOLERANCE_RELATIVE_METALLICITY], 1.0);
+  const double zsc = fmax(fabs(y[GAL_V_DISK_ABUNDANCES_GAS]) + fabs(y[GAL_V_DISK_ABUNDANCES_STELLAR]) +
+                              fabs(y[GAL_V_SPHEROID_ABUNDANCES_GAS]) + fabs(y[GAL_V_SPHEROID_ABUNDANCES_STELLAR]),
+                          zmin);
+  sc[GAL_V_DISK_ANGULAR_MOMENTUM] = ang;
+  sc[GAL_V_DISK_MASS_GAS] = mcomp;
+  sc[GAL_V_DISK_MASS_STELLAR] = mcomp;
+  sc[GAL_V_DISK_MASS_STELLAR_FORMED] = mcomp;
+  sc[GAL_V_DISK_FRACTION_MASS_RETAINED] = 1.0e-4 * y[GAL_V_DISK_FRACTION_MASS_RETAINED];
+  sc[GAL_V_DISK_ABUNDANCES_GAS] = zsc;
+  sc[GAL_V_DISK_ABUNDANCES_STELLAR] = zsc;
+  sc[GAL_V_SPHEROID_ANGULAR_MOMENTUM] = ang;
+  sc[GAL_V_SPHEROID_MASS_GAS] = mcomp;
+  sc[GAL_V_SPHEROID_MASS_STELLAR] = mcomp;
+  sc[GAL_V_SPHEROID_MASS_STELLAR_FORMED] = mcomp;
+  sc[GAL_V_SPHEROID_ABUNDANCES_GAS] = zsc;
+  sc[GAL_V_SPHEROID_ABUNDANCES_STELLAR] = zsc;
+  sc[GAL_V_BH_MASS] = fmax(fmax(1.0, 1.0e-4 * y[GAL_V_SPHEROID_MASS_STELLAR]), y[GAL_V_BH_MASS]);
+  sc[GAL_V_BH_SPIN] = 1.0;
+  sc[GAL_V_BH_RADIAL_POSITION] = fmax(1.0e-4 * H.r_gal, 1.0e-6);
+  sc[GAL_V_SPIN_ANGULAR_MOMENTUM] = mass * r_vir * v_vir * mrel;
+  sc[GAL_V_SATELLITE_BOUND_MASS] = mass * mrel;
+}
+
+// Post-step: standardPostStepProcessing (node_evolver/standard.F90:1160-1185)
+// with the disk/spheroid (disk/standard/_class.F90:506-660) and hot-halo
+// (hot_halo/standard/_class.F90:468-475) negative-value fixes.  The state
+// change is a GSL_CONTINUE: no evolver reset.  The state is addressed as
+// y[v*stride] so the same code serves a lane's LDS copy.
+__device__ __forceinline__ void component_post_step(double* __restrict__ y, int stride, int i_ms, int i_zs,
+                                                    int i_mg, int i_zg, int i_j, double rv) {
+#define PY(v) y[(v) * stride]
+  if (PY(i_mg) < 0.0) {
+    const double m = PY(i_mg) + PY(i_ms);
+    double j;
+    if (m == 0.0) { j = 0.0; PY(i_ms) = 0.0; PY(i_zs) = 0.0; }
+    else { j = PY(i_j) / m; if (j < 0.0) j = rv; }
+    PY(i_mg) = 0.0; PY(i_zg) = 0.0; PY(i_j) = j * PY(i_ms);
+  }
+  if (PY(i_ms) < 0.0) {
+    const double m = PY(i_mg) + PY(i_ms);
+    double j;
+    if (m == 0.0) { j = 0.0; PY(i_mg) = 0.0; PY(i_zg) = 0.0; }
+    else { j = PY(i_j) / m; if (j < 0.0) j = rv; }
+    PY(i_ms) = 0.0; PY(i_zs) = 0.0; PY(i_j) = j * PY(i_mg);
+  }
+  if (PY(i_j) < 0.0) PY(i_j) = 0.0;
+#undef PY
+}
+
+// The rarely taken fix-up is kept out of line; its arguments are passed by
+// value so that nothing of the caller's has to live in addressable memory.
+struct HaloConsts {
+  double om, ol, a3_norm, sinh_arg, rvir_coeff;
+};
+
+__device__ __noinline__ void post_step_fix(const double* __restrict__ P, HaloConsts hc, NodePar np, double t,
+                                           double* __restrict__ y, int stride) {
+  DerivedConstants D{};
+  D.om = hc.om;
+  D.ol = hc.ol;
+  D.a3_norm = hc.a3_norm;
+  D.sinh_arg = hc.sinh_arg;
+  D.rvir_coeff = hc.rvir_coeff;
+  const Halo H = halo(P, D, np, t, y[GAL_V_SPIN_ANGULAR_MOMENTUM * stride]);
+  const double rv = H.r_gal * H.v_vir;
+  component_post_step(y, stride, GAL_V_DISK_MASS_STELLAR, GAL_V_DISK_ABUNDANCES_STELLAR, GAL_V_DISK_MASS_GAS,
+                      GAL_V_DISK_ABUNDANCES_GAS, GAL_V_DISK_ANGULAR_MOMENTUM, rv);
+  component_post_step(y, stride, GAL_V_SPHEROID_MASS_STELLAR, GAL_V_SPHEROID_ABUNDANCES_STELLAR,
+                      GAL_V_SPHEROID_MASS_GAS, GAL_V_SPHEROID_ABUNDANCES_GAS, GAL_V_SPHEROID_ANGULAR_MOMENTUM, rv);
+  if (y[GAL_V_HOT_MASS * stride] < 0.0) y[GAL_V_HOT_MASS * stride] = 0.0;
+  if (y[GAL_V_HOT_OUTER_RADIUS * stride] < 0.0) y[GAL_V_HOT_OUTER_RADIUS * stride] = 0.0;
+}
+
+__device__ __forceinline__ void post_step(const double* __restrict__ P, const DerivedConstants& D, const NodePar& np,
+                                          int mode, double t, double* __restrict__ y, int stride) {
+  if (mode == GAL_PHYSICS_TEST) return;
+  const bool neg = y[GAL_V_DISK_MASS_GAS * stride] < 0.0 || y[GAL_V_DISK_MASS_STELLAR * stride] < 0.0 ||
+                   y[GAL_V_DISK_ANGULAR_MOMENTUM * stride] < 0.0 || y[GAL_V_SPHEROID_MASS_GAS * stride] < 0.0 ||
+                   y[GAL_V_SPHEROID_MASS_STELLAR * stride] < 0.0 ||
+                   y[GAL_V_SPHEROID_ANGULAR_MOMENTUM * stride] < 0.0 || y[GAL_V_HOT_MASS * stride] < 0.0 ||
+                   y[GAL_V_HOT_OUTER_RADIUS * stride] < 0.0;
+  if (neg) post_step_fix(P, HaloConsts{D.om, D.ol, D.a3_norm, D.sinh_arg, D.rvir_coeff}, np, t, y, stride);
+}
+
+}  // namespace gal

@@ -1,0 +1,168 @@
+"""Seeded synthetic node batches.
+
+``isolated_halos`` draws the batch of independent isolated-halo node ODE
+systems used for the throughput measurement: halo mass log-uniform in
+[1e9, 1e13] Msun, initial hot-gas fraction uniform in [0.05, 0.17], hot-gas
+metallicity log-uniform in [1e-4, 0.02], interval length uniform in
+[0.1, 2] Gyr, cosmology Omega_m=0.3, Omega_L=0.7, h=0.7.  Every other initial
+component value is drawn from the same generator so a seed fixes the batch.
+"""
+from __future__ import annotations
+
+import dataclasses
+import math
+
+import numpy as np
+
+from . import params as gp
+
+NV = gp.NVAR
+
+# State indices (enum gal_var of include/gal_ode.h).
+V = dict(
+    HOT_MASS=0, HOT_ABUNDANCES=1, HOT_ANGULAR_MOMENTUM=2, HOT_OUTFLOWED_MASS=3,
+    HOT_OUTFLOWED_ANGULAR_MOMENTUM=4, HOT_OUTFLOWED_ABUNDANCES=5, HOT_UNACCRETED_MASS=6,
+    HOT_UNACCRETED_ABUNDANCES=7, HOT_OUTER_RADIUS=8, HOT_STRIPPED_MASS=9, HOT_STRIPPED_ABUNDANCES=10,
+    HOT_ENERGY_RADIATED=11, DISK_MASS_STELLAR=12, DISK_MASS_STELLAR_FORMED=13,
+    DISK_FRACTION_MASS_RETAINED=14, DISK_ABUNDANCES_STELLAR=15, DISK_MASS_GAS=16, DISK_ABUNDANCES_GAS=17,
+    DISK_ANGULAR_MOMENTUM=18, SPHEROID_MASS_STELLAR=19, SPHEROID_MASS_STELLAR_FORMED=20,
+    SPHEROID_ABUNDANCES_STELLAR=21, SPHEROID_MASS_GAS=22, SPHEROID_ABUNDANCES_GAS=23,
+    SPHEROID_ANGULAR_MOMENTUM=24, BH_MASS=25, BH_SPIN=26, BH_RADIAL_POSITION=27,
+    SPIN_ANGULAR_MOMENTUM=28, SATELLITE_BOUND_MASS=29,
+)
+VAR_NAMES = [k for k, _ in sorted(V.items(), key=lambda kv: kv[1])]
+NP = dict(TIME_REFERENCE=0, MASS_REFERENCE=1, MASS_RATE=2, SPIN_RATE=3, IS_SATELLITE=4)
+
+# Masses and metal masses (the quantities compared per component).
+MASS_VARS = [V[k] for k in ("HOT_MASS", "HOT_OUTFLOWED_MASS", "HOT_UNACCRETED_MASS", "HOT_STRIPPED_MASS",
+                            "DISK_MASS_STELLAR", "DISK_MASS_GAS", "SPHEROID_MASS_STELLAR",
+                            "SPHEROID_MASS_GAS", "BH_MASS")]
+METAL_VARS = [V[k] for k in ("HOT_ABUNDANCES", "HOT_OUTFLOWED_ABUNDANCES", "HOT_UNACCRETED_ABUNDANCES",
+                             "HOT_STRIPPED_ABUNDANCES", "DISK_ABUNDANCES_STELLAR", "DISK_ABUNDANCES_GAS",
+                             "SPHEROID_ABUNDANCES_STELLAR", "SPHEROID_ABUNDANCES_GAS")]
+
+# Constants matching the engine (see galacticus_amd/csrc/constants.hpp).
+_PARSEC = 3.0856775814913673e16
+_MPC = 1.0e6 * _PARSEC
+_GYR = 1.0e9 * 3.15581497635456e7
+_MSUN = 1.98847e30
+_G_SI = 6.67430e-11
+G_INTERNAL = _G_SI * _MSUN / 1.0e6 / _MPC
+MPC_PER_KMS_GYR = _MPC / 1.0e3 / _GYR
+
+
+def cosmic_time_to_scale_factor(t, om=0.3, ol=0.7, h0=70.0):
+    h0g = h0 / MPC_PER_KMS_GYR
+    return (om / ol) ** (1.0 / 3.0) * np.sinh(1.5 * math.sqrt(ol) * h0g * np.asarray(t)) ** (2.0 / 3.0)
+
+
+def scale_factor_to_cosmic_time(a, om=0.3, ol=0.7, h0=70.0):
+    h0g = h0 / MPC_PER_KMS_GYR
+    a = np.asarray(a, dtype=np.float64)
+    return np.arcsinh((a / (om / ol) ** (1.0 / 3.0)) ** 1.5) / (1.5 * math.sqrt(ol) * h0g)
+
+
+def virial(mass, t, om=0.3, ol=0.7, h0=70.0):
+    """(r_vir [Mpc], V_vir [km/s]) with the engine's Bryan-Norman contrast."""
+    a = cosmic_time_to_scale_factor(t, om, ol, h0)
+    a3 = a ** 3
+    rho_crit0 = 3.0 * h0 * h0 / (8.0 * math.pi * G_INTERNAL)
+    rho_mean = om * rho_crit0 / a3
+    om_a = (om / a3) / (om / a3 + ol)
+    x = om_a - 1.0
+    delta_m = (18.0 * math.pi ** 2 + 82.0 * x - 39.0 * x * x) / om_a
+    r = np.cbrt(3.0 * mass / (4.0 * math.pi * delta_m * rho_mean))
+    v = np.sqrt(G_INTERNAL * mass / r)
+    return r, v
+
+
+@dataclasses.dataclass
+class NodeBatch:
+    y: np.ndarray          # (30, n)
+    node_par: np.ndarray   # (5, n)
+    t_start: np.ndarray
+    t_end: np.ndarray
+    h_step: np.ndarray
+
+    @property
+    def n(self):
+        return self.y.shape[1]
+
+    @property
+    def node_gyr(self):
+        return float(np.sum(np.abs(self.t_end - self.t_start)))
+
+
+def isolated_halos(n: int, seed: int = 0, params=None) -> NodeBatch:
+    p = gp.physics_params() if params is None else params
+    om, ol, h0 = p[gp.P["OmegaMatter"]], p[gp.P["OmegaDarkEnergy"]], p[gp.P["HubbleConstant"]]
+    fb = p[gp.P["OmegaBaryon"]] / om
+    rng = np.random.Generator(np.random.PCG64(seed))
+    mass = 10.0 ** rng.uniform(9.0, 13.0, n)
+    f_hot = rng.uniform(0.05, 0.17, n)
+    metal = 10.0 ** rng.uniform(-4.0, math.log10(0.02), n)
+    dt = rng.uniform(0.1, 2.0, n)
+    t0 = rng.uniform(2.0, 11.5, n)
+    spin = np.clip(np.exp(rng.normal(math.log(0.035), 0.5, n)), 0.01, 0.1)
+    growth = rng.uniform(0.0, 0.1, n)
+    f_out = rng.uniform(0.0, 0.02, n)
+    f_dg = rng.uniform(0.01, 0.1, n)
+    f_ds = rng.uniform(0.0, 0.1, n)
+    f_sg = rng.uniform(0.0, 0.02, n)
+    f_ss = rng.uniform(0.0, 0.1, n)
+    bh_spin = rng.uniform(0.0, 0.9, n)
+    r_vir, v_vir = virial(mass, t0, om, ol, h0)
+    j_spin = spin * mass * v_vir * r_vir
+    r_disk = p[gp.P["galacticStructureSolver.factor"]] * r_vir * spin
+    y = np.zeros((NV, n))
+    mb = fb * mass
+    y[V["HOT_MASS"]] = f_hot * mass
+    y[V["HOT_ABUNDANCES"]] = metal * y[V["HOT_MASS"]]
+    y[V["HOT_ANGULAR_MOMENTUM"]] = y[V["HOT_MASS"]] * j_spin / mass
+    y[V["HOT_OUTFLOWED_MASS"]] = f_out * mb
+    y[V["HOT_OUTFLOWED_ABUNDANCES"]] = metal * y[V["HOT_OUTFLOWED_MASS"]]
+    y[V["HOT_OUTFLOWED_ANGULAR_MOMENTUM"]] = y[V["HOT_OUTFLOWED_MASS"]] * j_spin / mass
+    y[V["HOT_OUTER_RADIUS"]] = r_vir
+    y[V["DISK_MASS_GAS"]] = f_dg * mb
+    y[V["DISK_MASS_STELLAR"]] = f_ds * mb
+    y[V["DISK_MASS_STELLAR_FORMED"]] = y[V["DISK_MASS_STELLAR"]]
+    y[V["DISK_FRACTION_MASS_RETAINED"]] = 1.0
+    y[V["DISK_ABUNDANCES_GAS"]] = metal * y[V["DISK_MASS_GAS"]]
+    y[V["DISK_ABUNDANCES_STELLAR"]] = metal * y[V["DISK_MASS_STELLAR"]]
+    y[V["DISK_ANGULAR_MOMENTUM"]] = (y[V["DISK_MASS_GAS"]] + y[V["DISK_MASS_STELLAR"]]) * 2.0 * r_disk * v_vir
+    y[V["SPHEROID_MASS_GAS"]] = f_sg * mb
+    y[V["SPHEROID_MASS_STELLAR"]] = f_ss * mb
+    y[V["SPHEROID_MASS_STELLAR_FORMED"]] = y[V["SPHEROID_MASS_STELLAR"]]
+    y[V["SPHEROID_ABUNDANCES_GAS"]] = metal * y[V["SPHEROID_MASS_GAS"]]
+    y[V["SPHEROID_ABUNDANCES_STELLAR"]] = metal * y[V["SPHEROID_MASS_STELLAR"]]
+    y[V["SPHEROID_ANGULAR_MOMENTUM"]] = (y[V["SPHEROID_MASS_GAS"]] + y[V["SPHEROID_MASS_STELLAR"]]) * 0.1 * r_disk * v_vir
+    y[V["BH_MASS"]] = 100.0 + 1.0e-3 * y[V["SPHEROID_MASS_STELLAR"]]
+    y[V["BH_SPIN"]] = bh_spin
+    y[V["SPIN_ANGULAR_MOMENTUM"]] = j_spin
+    y[V["SATELLITE_BOUND_MASS"]] = mass
+    npar = np.zeros((gp.NPARAM_NODE, n))
+    npar[NP["TIME_REFERENCE"]] = t0
+    npar[NP["MASS_REFERENCE"]] = mass
+    npar[NP["MASS_RATE"]] = growth * mass
+    npar[NP["SPIN_RATE"]] = growth * j_spin
+    npar[NP["IS_SATELLITE"]] = 0.0
+    return NodeBatch(y, npar, t0, t0 + dt, -np.ones(n))
+
+
+def closed_box(timescale=0.25, outflow=0.0, recycled=0.4, yield_=0.025):
+    """The reference's closed/leaky-box single-node known answers
+    (testSuite/test-reproducibility.py:47-68, testSuite/parameters/reproducibility/{closed,leaky}Box*.xml):
+    one disk with 1e11 Msun of metal-free gas at t=12.47 Gyr evolved to 13.47 Gyr."""
+    p = gp.physics_params(physicsMode=gp.PHYSICS_BOX, OmegaMatter=0.3, OmegaDarkEnergy=0.7, OmegaBaryon=0.05,
+                          HubbleConstant=70.0, recycledFraction=recycled, metalYield=yield_,
+                          **{"starFormationTimescale.fixed.timescale": timescale,
+                             "stellarFeedbackOutflows.fixed.fraction": outflow})
+    y = np.zeros((NV, 1))
+    y[V["DISK_MASS_GAS"], 0] = 1.0e11
+    y[V["DISK_FRACTION_MASS_RETAINED"], 0] = 1.0
+    y[V["SPIN_ANGULAR_MOMENTUM"], 0] = 1.635e12
+    y[V["SATELLITE_BOUND_MASS"], 0] = 1.0e12
+    npar = np.zeros((gp.NPARAM_NODE, 1))
+    npar[NP["TIME_REFERENCE"], 0] = 12.47
+    npar[NP["MASS_REFERENCE"], 0] = 1.0e12
+    return p, NodeBatch(y, npar, np.array([12.47]), np.array([13.47]), -np.ones(1))

@@ -1,0 +1,192 @@
+"""GPU parity: the HIP engine against the CPU oracle and the reference's known answers.
+
+All calls go through the C ABI (include/gal_ode.h) via galacticus_amd.evolver.
+Bars:
+  * controller (cscal2.c:93-169): decision identical; h within 4 ulp (the
+    device pow and glibc's pow differ in the last bits of rmax^(-1/ord)).
+  * RHS: |f_gpu - f_oracle| <= 1e-9 * (|f_oracle| + rate scale) elementwise.
+  * evolution: every node within the ODE tolerance of the oracle
+    (|dy| <= 10 * (eps_rel*|y| + eps_abs*scale)), median relative
+    difference of the masses below 1e-9, identical statuses.
+  * known answers: the reference's closed/leaky box numbers at its 1%
+    tolerance (testSuite/test-reproducibility.py:47-68) and the ODE solver
+    closed forms at absTol 1e-6 / relTol 5e-6 (source/tests/ODE_solver.F90:73-104).
+"""
+import math
+
+import numpy as np
+import pytest
+
+import oracle
+from galacticus_amd import params as gp
+from galacticus_amd import workloads as wl
+
+pytestmark = pytest.mark.gpu
+
+
+@pytest.fixture(scope="module")
+def torch_cuda():
+    import torch
+    assert torch.cuda.is_available(), "GPU tests need a GPU"
+    return torch
+
+
+def _evolver(n, physics=None, **kw):
+    from galacticus_amd.evolver import NodeEvolverBatched
+    return NodeEvolverBatched(n, physics, **kw)
+
+
+def test_library_loads_and_device(torch_cuda):
+    from galacticus_amd import _lib
+    import ctypes
+    L = _lib.load()
+    nd, ncu = ctypes.c_int(), ctypes.c_int()
+    assert L.gal_ode_device_check(ctypes.byref(nd), ctypes.byref(ncu)) == 0
+    print("devices", nd.value, "CUs", ncu.value, flush=True)
+    assert ncu.value >= 64
+
+
+def test_controller_matches_oracle(torch_cuda):
+    torch = torch_cuda
+    from galacticus_amd.evolver import hadjust_device
+    rng = np.random.default_rng(1)
+    dim, n = 30, 4096
+    y = rng.normal(0, 1, (dim, n)) * 10.0 ** rng.uniform(-3, 3, (dim, n))
+    scale = 10.0 ** rng.uniform(-2, 2, (dim, n))
+    # error magnitudes spanning every branch: rmax >> 1.1, in (0.5,1.1), < 0.5, tiny
+    lev = 10.0 ** rng.uniform(-8, 1, n)
+    yerr = rng.normal(0, 1, (dim, n)) * (1e-3 * np.abs(y) + 1e-6 * scale) * lev[None, :] * 3.0
+    yp = rng.normal(0, 1, (dim, n))
+    nonneg = (rng.random((dim, n)) < 0.3).astype(np.int32)
+    h = 10.0 ** rng.uniform(-4, 0, n)
+    # a few zero-scale / zero-error entries (NaN ratio ignored by the max)
+    scale[0, :16] = 0.0
+    yerr[0, :16] = 0.0
+    y[0, :16] = 0.0
+    dev = lambda a: torch.from_numpy(np.ascontiguousarray(a)).cuda()
+    hd = dev(h.copy())
+    res = hadjust_device(dev(y), dev(yerr), dev(yp), dev(scale), dev(nonneg), hd, 4, 1e-6, 1e-3, 1.0, 0.0)
+    torch.cuda.synchronize()
+    res = res.cpu().numpy()
+    hg = hd.cpu().numpy()
+    dec = np.zeros(n, dtype=int)
+    ho = np.zeros(n)
+    for i in range(n):
+        dec[i], ho[i] = oracle.hadjust(4, 1e-6, 1e-3, 1.0, 0.0, y[:, i], yerr[:, i], yp[:, i], scale[:, i],
+                                       nonneg[:, i], h[i])
+    assert np.array_equal(res, dec)
+    assert set(np.unique(dec)) == {-1, 0, 1}
+    ulp = np.abs(hg - ho) / np.spacing(np.abs(ho))
+    assert ulp.max() <= 4.0, ulp.max()
+
+
+def test_rhs_matches_oracle(torch_cuda, cooling_table):
+    torch = torch_cuda
+    b = wl.isolated_halos(512, seed=3)
+    p = gp.physics_params()
+    ev = _evolver(512, p, table=cooling_table)
+    rng = np.random.default_rng(4)
+    t = b.t_start + rng.uniform(0, 1, b.n) * (b.t_end - b.t_start)
+    dev = lambda a: torch.from_numpy(np.ascontiguousarray(a)).cuda()
+    fg = ev.rhs_device(dev(t), dev(b.y), dev(b.node_par)).cpu().numpy()
+    fo = oracle.rhs(t, b.y, b.node_par, p, cooling_table, root_tol_rel=1e-15)
+    sc = oracle.scales(b.t_start, b.y, b.node_par, p)
+    tol = 1e-9 * (np.abs(fo) + sc)
+    bad = np.abs(fg - fo) > tol
+    assert not bad.any(), [(v, int(np.sum(bad[v]))) for v in range(30) if bad[v].any()]
+
+
+@pytest.mark.parametrize("case", ["closedBox", "leakyBox"])
+def test_box_known_answers(torch_cuda, cooling_table, case):
+    """testSuite/test-reproducibility.py:47-68 (closed box tau=0.25; leaky box tau=0.5, beta=1)."""
+    if case == "closedBox":
+        p, b = wl.closed_box(0.25, 0.0)
+        expect = dict(DISK_MASS_GAS=9.0717953e9, DISK_MASS_STELLAR=9.0928205e10, DISK_ABUNDANCES_GAS=9.0717953e8,
+                      DISK_ABUNDANCES_STELLAR=2.8814957e9)
+        tols = dict(DISK_MASS_GAS=1e-2)
+    else:
+        p, b = wl.closed_box(0.5, 1.0)
+        expect = dict(DISK_MASS_GAS=4.0762204e9, DISK_MASS_STELLAR=3.5971417e10, DISK_ABUNDANCES_GAS=2.03811e8,
+                      DISK_ABUNDANCES_STELLAR=4.85624e8)
+        tols = dict(DISK_MASS_GAS=1.1e-2)
+    ev = _evolver(1, p, table=cooling_table, odeToleranceAbsolute=1e-6, odeToleranceRelative=1e-3)
+    y, h, sc, st, cnt = ev.evolve(b.y, b.node_par, b.t_start, b.t_end, b.h_step)
+    assert st[0] == 0
+    for k, v in expect.items():
+        got = y[wl.V[k], 0]
+        assert abs(got / v - 1.0) <= tols.get(k, 1e-2), (k, got, v)
+    # and the same answer as the oracle's restatement, step for step
+    yo, ho, _, sto, cnto = oracle.evolve(b.y, b.node_par, b.t_start, b.t_end, b.h_step, p, cooling_table, 1e-6,
+                                         1e-3, gp.FLAG_REUSE_STEP_SIZE)
+    assert np.array_equal(cnt[0], cnto[0]) and np.array_equal(cnt[1], cnto[1])
+    np.testing.assert_allclose(y, yo, rtol=1e-12, atol=1e-3)
+
+
+def test_ode_solver_closed_forms(torch_cuda, cooling_table):
+    """source/tests/ODE_solver.F90:73-104: y'=sin(x) reversed from x to 0, and y''=-y from 0 to x,
+    RKCK, tolerances 1e-9, scale 1."""
+    p = gp.physics_params(physicsMode=gp.PHYSICS_TEST)
+    xs = np.arange(1.0, 11.0)
+    n = 2 * len(xs)
+    y = np.zeros((30, n))
+    t0 = np.zeros(n)
+    t1 = np.zeros(n)
+    for i, x in enumerate(xs):
+        y[0, i] = 1.0 - math.cos(x)      # reversed sine
+        t0[i], t1[i] = x, 0.0
+        j = len(xs) + i
+        y[1, j], y[2, j] = 1.0, 0.0      # oscillator
+        t0[j], t1[j] = 0.0, x
+    npar = np.zeros((5, n))
+    sc = np.ones((30, n))
+    ev = _evolver(n, p, table=cooling_table, odeToleranceAbsolute=1e-9, odeToleranceRelative=1e-9,
+                  reuseODEStepSize=False, scales_provided=True)
+    yg, h, _, st, cnt = ev.evolve(y, npar, t0, t1, -np.ones(n), scale=sc)
+    assert (st == 0).all()
+    for i, x in enumerate(xs):
+        assert abs(yg[0, i] - 0.0) <= 1e-6
+        j = len(xs) + i
+        assert abs(yg[1, j] - math.cos(x)) <= 1e-6 * abs(math.cos(x)) + 1e-9
+        assert abs(yg[2, j] + math.sin(x)) <= 1e-6 * abs(math.sin(x)) + 1e-9
+        # identical step counts to the oracle's restatement
+        st1, yo, ho, co = oracle.test_ode_solve(1, x, 0.0, [1.0 - math.cos(x)], 1e-9, 1e-9)
+        assert cnt[0, i] == co[0] and cnt[1, i] == co[1]
+        st2, yo2, ho2, co2 = oracle.test_ode_solve(2, 0.0, x, [1.0, 0.0], 1e-9, 1e-9)
+        assert cnt[0, j] == co2[0] and cnt[1, j] == co2[1]
+        assert abs(yg[0, i] - yo[0]) <= 1e-12
+        assert np.allclose(yg[1:3, j], yo2, rtol=1e-11, atol=1e-14)
+
+
+def _compare_evolution(yg, yo, sc, eps_abs, eps_rel):
+    tol = 10.0 * (eps_rel * np.abs(yo) + eps_abs * sc)
+    bad = np.abs(yg - yo) > tol
+    masses = wl.MASS_VARS
+    rel = np.abs(yg[masses] - yo[masses]) / np.maximum(np.abs(yo[masses]), 1.0)
+    return bad, float(np.median(rel)), float(rel.max())
+
+
+@pytest.mark.parametrize("reuse", [True, False])
+def test_evolve_matches_oracle(torch_cuda, cooling_table, reuse):
+    n = 3000
+    b = wl.isolated_halos(n, seed=11)
+    p = gp.physics_params()
+    flags = gp.FLAG_REUSE_STEP_SIZE if reuse else 0
+    print(f"evolve parity n={n} reuse={reuse}", flush=True)
+    ev = _evolver(n, p, table=cooling_table, odeToleranceAbsolute=1e-6, odeToleranceRelative=1e-3,
+                  reuseODEStepSize=reuse)
+    hin = b.h_step.copy()
+    if reuse:
+        hin = (b.t_end - b.t_start) * 0.05
+    yg, hg, scg, stg, cntg = ev.evolve(b.y, b.node_par, b.t_start, b.t_end, hin)
+    yo, ho, sco, sto, cnto = oracle.evolve(b.y, b.node_par, b.t_start, b.t_end, hin, p, cooling_table, 1e-6, 1e-3,
+                                           flags, root_tol_rel=1e-15, nthreads=8)
+    assert np.array_equal(stg, sto)
+    assert (stg == 0).all()
+    np.testing.assert_allclose(scg, sco, rtol=1e-13)
+    bad, med, mx = _compare_evolution(yg, yo, sco, 1e-6, 1e-3)
+    same_steps = np.mean(cntg[0] == cnto[0])
+    print(f"median rel {med:.3e} max rel {mx:.3e} same-step-count fraction {same_steps:.4f}", flush=True)
+    assert not bad.any(), int(bad.sum())
+    assert med < 1e-9
+    assert same_steps > 0.95
+    np.testing.assert_allclose(hg[cntg[0] == cnto[0]], ho[cntg[0] == cnto[0]], rtol=1e-6)
