@@ -44,6 +44,8 @@
 #include "physics.hpp"
 #include "integrator.hpp"
 
+#include <hipcub/device/device_radix_sort.hpp>
+
 using namespace gal;
 
 namespace {
@@ -407,6 +409,44 @@ __global__ void hadjust_kernel(int n, int dim, int ord, double eps_abs, double e
   h[i] = hh;
 }
 
+// Stiffness probe for the claim order: one RHS evaluation per node at its
+// start point; cost key = |t_end - t_start| / sqrt(tau), tau = the shortest
+// e-folding time min_v (|y_v| + scale_v) / |dy_v/dt|.  Only the order in
+// which lanes claim nodes depends on it, never a result.
+__global__ void cost_probe_kernel(int n, int ld, const double* __restrict__ y, const double* __restrict__ npa,
+                                  const double* __restrict__ t_start, const double* __restrict__ t_end,
+                                  const double* __restrict__ scale_in, const double* __restrict__ P,
+                                  const double* __restrict__ tab, DerivedConstants D, CoolingDesc cd, int mode,
+                                  int scales_given, float* __restrict__ key, int* __restrict__ index) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  double yy[kNV], f[kNV], sc[kNV];
+#pragma unroll
+  for (int v = 0; v < kNV; ++v) yy[v] = y[v * (size_t)ld + i];
+  const NodePar np = load_node_par(npa, ld, i);
+  const double t0 = t_start[i];
+  if (scales_given) {
+#pragma unroll
+    for (int v = 0; v < kNV; ++v) sc[v] = scale_in[v * (size_t)ld + i];
+  } else if (mode == GAL_PHYSICS_TEST) {
+#pragma unroll
+    for (int v = 0; v < kNV; ++v) sc[v] = 1.0;
+  } else {
+    scales(P, D, np, t0, yy, sc);
+  }
+  rhs(P, D, tab, cd, mode, np, t0, yy, f);
+  double tau = 1.0e30;
+#pragma unroll
+  for (int v = 0; v < kNV; ++v) {
+    const double r = fabs(f[v]);
+    const double tv = (fabs(yy[v]) + fabs(sc[v])) / r;
+    tau = (r > 0.0 && tv < tau) ? tv : tau;
+  }
+  const double k = fabs(t_end[i] - t0) * rsqrt(fmax(tau, 1.0e-30));
+  key[i] = (float)fmin(k, 3.0e38);
+  index[i] = i;
+}
+
 template <int BLOCK>
 constexpr size_t lds_bytes() {
   return sizeof(double) * kLdsVectors * kNV * BLOCK;
@@ -432,6 +472,11 @@ struct gal_ode_batch {
   double* d_table = nullptr;
   unsigned int* d_work = nullptr;
   int* d_order = nullptr;
+  int* d_index = nullptr;
+  float* d_key = nullptr;
+  float* d_key_sorted = nullptr;
+  void* d_sort_temp = nullptr;
+  size_t sort_temp_bytes = 0;
   bool have_physics = false, have_table = false;
   // staging for the host-pointer entry point
   double *d_y = nullptr, *d_np = nullptr, *d_t0 = nullptr, *d_t1 = nullptr, *d_h = nullptr, *d_scale = nullptr;
@@ -493,6 +538,17 @@ static void derive_constants(gal_ode_batch* h) {
   D.unit_energy = kMassSolar * kKilo * kKilo / kErgs / kGigaYear;
   const double mpc_cm = kHecto * kMegaParsec;
   D.nh_coeff = kMassSolar / kMassHydrogenAtom / (mpc_cm * mpc_cm * mpc_cm);
+  D.inv_eff_sf_disk = p[GAL_P_SF_DISK_EFFICIENCY] != 0.0 ? 1.0 / p[GAL_P_SF_DISK_EFFICIENCY] : 0.0;
+  D.inv_eff_sf_sph = p[GAL_P_SF_SPHEROID_EFFICIENCY] != 0.0 ? 1.0 / p[GAL_P_SF_SPHEROID_EFFICIENCY] : 0.0;
+  D.inv_tmin_fb_disk = 1.0 / p[GAL_P_FB_DISK_TIMESCALE_MINIMUM];
+  D.inv_tmin_fb_sph = 1.0 / p[GAL_P_FB_SPHEROID_TIMESCALE_MINIMUM];
+  D.inv_core = 1.0 / p[GAL_P_CORE_RADIUS_OVER_VIRIAL];
+  D.inv_cs2_sph = D.cs2_sph > 0.0 ? 1.0 / D.cs2_sph : 0.0;
+  const double bhl_unit = kKilo * kGigaYear / kMegaParsec * 4.0 * kPi;
+  D.bhl_sph_coeff = D.cs2_sph > 0.0 ? p[GAL_P_BH_ENHANCEMENT_SPHEROID] * bhl_unit * kGravity * kGravity /
+                                          (D.cs2_sph * std::sqrt(D.cs2_sph))
+                                    : 0.0;
+  D.jeans_coeff = D.cs2_sph > 0.0 ? kGravity / D.cs2_sph : 0.0;
 }
 
 extern "C" {
@@ -534,9 +590,16 @@ int gal_ode_batch_create(int n_max, double tol_abs, double tol_rel, int flags, g
   if (hipMalloc(&h->d_params, sizeof(double) * GAL_NPARAM) != hipSuccess ||
       hipMalloc(&h->d_table, sizeof(double) * GAL_COOLING_NT_MAX * GAL_COOLING_NZ_MAX) != hipSuccess ||
       hipMalloc(&h->d_work, sizeof(unsigned int)) != hipSuccess ||
-      hipMalloc(&h->d_order, sizeof(int) * nm) != hipSuccess) {
-    hipFree(h->d_params); hipFree(h->d_table); hipFree(h->d_work); hipFree(h->d_order);
-    delete h;
+      hipMalloc(&h->d_order, sizeof(int) * nm) != hipSuccess || hipMalloc(&h->d_index, sizeof(int) * nm) != hipSuccess ||
+      hipMalloc(&h->d_key, sizeof(float) * nm) != hipSuccess ||
+      hipMalloc(&h->d_key_sorted, sizeof(float) * nm) != hipSuccess) {
+    gal_ode_batch_destroy(h);
+    return GAL_E_NOMEM;
+  }
+  if (hipcub::DeviceRadixSort::SortPairsDescending(nullptr, h->sort_temp_bytes, h->d_key, h->d_key_sorted,
+                                                   h->d_index, h->d_order, (int)nm) != hipSuccess ||
+      hipMalloc(&h->d_sort_temp, std::max<size_t>(h->sort_temp_bytes, 16)) != hipSuccess) {
+    gal_ode_batch_destroy(h);
     return GAL_E_NOMEM;
   }
   static bool attrs_set = false;
@@ -569,6 +632,10 @@ int gal_ode_batch_destroy(gal_ode_batch* h) {
   hipFree(h->d_table);
   hipFree(h->d_work);
   hipFree(h->d_order);
+  hipFree(h->d_index);
+  hipFree(h->d_key);
+  hipFree(h->d_key_sorted);
+  hipFree(h->d_sort_temp);
   free_staging(h);
   delete h;
   return GAL_OK;
@@ -580,7 +647,8 @@ int gal_ode_batch_set_physics(gal_ode_batch* h, const double* params, int n_para
   if (mode < GAL_PHYSICS_STANDARD || mode > GAL_PHYSICS_TEST) return GAL_E_INVALID;
   if (!(params[GAL_P_OMEGA_MATTER] > 0.0) || !(params[GAL_P_OMEGA_DARK_ENERGY] > 0.0) ||
       !(params[GAL_P_HUBBLE_CONSTANT] > 0.0) || !(params[GAL_P_FB_DISK_VELOCITY] > 0.0) ||
-      !(params[GAL_P_FB_SPHEROID_VELOCITY] > 0.0))
+      !(params[GAL_P_FB_SPHEROID_VELOCITY] > 0.0) || !(params[GAL_P_FB_DISK_TIMESCALE_MINIMUM] > 0.0) ||
+      !(params[GAL_P_FB_SPHEROID_TIMESCALE_MINIMUM] > 0.0) || !(params[GAL_P_CORE_RADIUS_OVER_VIRIAL] > 0.0))
     return GAL_E_INVALID;
   std::memcpy(h->params, params, sizeof(double) * GAL_NPARAM);
   derive_constants(h);
@@ -668,7 +736,19 @@ int gal_ode_batch_evolve_d(gal_ode_batch* h, int n, int ld, double* y, const dou
   int rc = make_args(h, n, ld, y, node_par, t_start, t_end, h_step, scale, status, counters, a);
   if (rc != GAL_OK) return rc;
   if (n == 0) return GAL_OK;
-  return launch_evolve(h, a, (hipStream_t)stream);
+  hipStream_t s = (hipStream_t)stream;
+  if (h->flags & GAL_FLAG_COST_ORDER) {
+    const int threads = 256;
+    hipLaunchKernelGGL(cost_probe_kernel, dim3((n + threads - 1) / threads), dim3(threads), 0, s, n, ld, y, node_par,
+                       t_start, t_end, (const double*)scale, (const double*)h->d_params, (const double*)h->d_table,
+                       h->D, h->cd, a.mode, (h->flags & GAL_FLAG_SCALES_PROVIDED) ? 1 : 0, h->d_key, h->d_index);
+    HIP_TRY(hipGetLastError());
+    size_t tb = h->sort_temp_bytes;
+    HIP_TRY(hipcub::DeviceRadixSort::SortPairsDescending(h->d_sort_temp, tb, h->d_key, h->d_key_sorted, h->d_index,
+                                                         h->d_order, n, 0, 32, s));
+    a.order = h->d_order;
+  }
+  return launch_evolve(h, a, s);
 }
 
 int gal_ode_batch_evolve_h(gal_ode_batch* h, int n, double* y, const double* node_par, const double* t_start,

@@ -55,7 +55,24 @@ struct DerivedConstants {
   double cool_k_pre;      // (dof/2) k / ergs / Gyr
   double unit_energy;     // Msun (km/s)^2 / Gyr -> erg/s
   double nh_coeff;        // Msun / m_H / (Mpc in cm)^3
+  double inv_eff_sf_disk, inv_eff_sf_sph;        // 1/efficiency (0 when efficiency is 0)
+  double inv_tmin_fb_disk, inv_tmin_fb_sph;      // 1/timescaleOutflowFractionalMinimum
+  double inv_core;                               // 1/coreRadiusOverVirialRadius
+  double inv_cs2_sph;                            // 1/cs^2 at the spheroid accretion temperature
+  double bhl_sph_coeff;                          // enhancement * 4 pi G^2 (unit) / cs^3, spheroid mode
+  double jeans_coeff;                            // G / cs^2 at the spheroid accretion temperature
 };
+
+// Reciprocal to full double precision: hardware estimate plus two
+// Newton-Raphson corrections (Markstein 1990).  Callers only pass finite,
+// non-zero values or select the result away.
+__device__ __forceinline__ double rcp(double x) {
+  double r = __builtin_amdgcn_rcp(x);
+  double e = fma(-x, r, 1.0);
+  r = fma(r, e, r);
+  e = fma(-x, r, 1.0);
+  return fma(r, e, r);
+}
 
 // Cooling table descriptor (table itself in global memory).
 struct CoolingDesc {
@@ -81,7 +98,7 @@ __device__ __forceinline__ double pow_half(double x, double sqrt_x, int n2) {
     if (k & 1) r *= b;
     b *= b;
   }
-  return n2 < 0 ? 1.0 / r : r;
+  return n2 < 0 ? rcp(r) : r;
 }
 
 // x^alpha: half-integer fast path, else exp(alpha ln x).
@@ -93,12 +110,17 @@ __device__ __forceinline__ double pow_param(double x, double sqrt_x, double ln_x
 // Bilinear interpolation of ln(Lambda) in (ln T, ln Z) with "fix"
 // extrapolation: cieFileInterpolatingFactors / cieFileInterpolate
 // (cooling/cooling_function/CIE_file.F90:665-735).  Grids are uniform, so
-// the bracketing index is computed directly instead of by bisection.
-__device__ __forceinline__ double cooling_function(const double* __restrict__ tab, const CoolingDesc& c,
-                                                   double temperature, double z_solar) {
-  const double T = fmin(fmax(temperature, c.t_min), c.t_max);
-  const double Z = fmin(fmax(z_solar, c.z_min), c.z_max);
-  const double lt = log(T), lz = log(Z);
+// the bracketing index is computed directly instead of by bisection.  The
+// logarithms of the clamped temperature and metallicity are passed in so
+// that they can be computed as early as their inputs are known.
+__device__ __forceinline__ double clamp_temperature(const CoolingDesc& c, double temperature) {
+  return fmin(fmax(temperature, c.t_min), c.t_max);
+}
+__device__ __forceinline__ double clamp_metallicity(const CoolingDesc& c, double z_solar) {
+  return fmin(fmax(z_solar, c.z_min), c.z_max);
+}
+__device__ __forceinline__ double cooling_function_ln(const double* __restrict__ tab, const CoolingDesc& c, double lt,
+                                                      double lz) {
   int it = (int)((lt - c.ln_t_min) * c.inv_d_ln_t);
   int iz = (int)((lz - c.ln_z_min) * c.inv_d_ln_z);
   it = it < 0 ? 0 : (it > c.n_t - 2 ? c.n_t - 2 : it);
@@ -122,20 +144,22 @@ __device__ __forceinline__ void isco(double a, double& energy, double& ang_mom) 
   const double a2 = sqrt(3.0 * a * a + a1 * a1);
   const double r = 3.0 + a2 - sqrt((3.0 - a1) * (3.0 + a1 + 2.0 * a2));
   const double sr = sqrt(r);
-  const double inv_den = 1.0 / (r * sqrt(r * r - 3.0 * r + 2.0 * a * sr));
+  const double inv_den = rcp(r) * rsqrt(r * r - 3.0 * r + 2.0 * a * sr);
   energy = (r * r - 2.0 * r + a * sr) * inv_den;
   ang_mom = (r * r - 2.0 * a * sr + a * a) * sr * inv_den;
 }
 
+// Hernquist density of mass M with scale radius a at r (zero for
+// non-positive arguments).
 __device__ __forceinline__ double hernquist_density(double mass, double scale, double r) {
-  if (!(mass > 0.0) || !(scale > 0.0) || !(r > 0.0)) return 0.0;
   const double rs = r + scale;
-  return mass * scale / (2.0 * kPi * r * rs * rs * rs);
+  const double rho = mass * scale * (1.0 / (2.0 * kPi)) * rcp(r * rs * rs * rs);
+  return (mass > 0.0 && scale > 0.0 && r > 0.0) ? rho : 0.0;
 }
 
 // Halo quantities shared by the RHS, the scale rules and the post-step.
 struct Halo {
-  double mass, a3, r_vir, v_vir, inv_v, t_dyn, lambda, r_gal;
+  double mass, a3, r_vir, inv_r_vir, v_vir, inv_v, t_dyn, inv_t_dyn, lambda, r_gal;
 };
 
 __device__ __forceinline__ Halo halo(const double* __restrict__ P, const DerivedConstants& D, const NodePar& np,
@@ -143,25 +167,35 @@ __device__ __forceinline__ Halo halo(const double* __restrict__ P, const Derived
   Halo h;
   const double mrate = np.satellite ? 0.0 : np.m_rate;
   h.mass = np.m_ref + mrate * (t - np.t_ref);
-  const double s = sinh(D.sinh_arg * t);
+  // a^3 = (Omega_m/Omega_L) sinh^2(x), sinh(x) = (e^x - e^-x)/2
+  const double ex = exp(D.sinh_arg * t);
+  const double s = 0.5 * (ex - rcp(ex));
   h.a3 = D.a3_norm * s * s;
-  const double om_a3 = D.om / h.a3;
+  const double om_a3 = D.om * rcp(h.a3);
   const double e2 = om_a3 + D.ol;
-  const double om_a = om_a3 / e2;
+  const double inv_e2 = rcp(e2);
+  const double om_a = om_a3 * inv_e2;
   const double x = om_a - 1.0;
   const double delta_c = 18.0 * kPi * kPi + 82.0 * x - 39.0 * x * x;
-  h.r_vir = cbrt(D.rvir_coeff * h.mass / (delta_c * e2));
-  const double u = kGravity * h.mass / h.r_vir;
+  h.r_vir = cbrt(D.rvir_coeff * h.mass * inv_e2 * rcp(delta_c));
+  h.inv_r_vir = rcp(h.r_vir);
+  const double u = kGravity * h.mass * h.inv_r_vir;
   h.inv_v = rsqrt(u);
   h.v_vir = u * h.inv_v;
   h.t_dyn = kMpcPerKmsToGyr * h.r_vir * h.inv_v;
+  h.inv_t_dyn = h.v_vir * h.inv_r_vir * (1.0 / kMpcPerKmsToGyr);
   const double jscale = h.mass * h.v_vir * h.r_vir;
-  h.lambda = jscale > 0.0 ? spin_j / jscale : 0.0;
+  h.lambda = jscale > 0.0 ? spin_j * rcp(jscale) : 0.0;
   h.r_gal = P[GAL_P_RADIUS_FACTOR] * h.r_vir * h.lambda;  // fixed.F90:318-325
   return h;
 }
 
 // ---- the RHS -------------------------------------------------------------
+// Written as straight-line dataflow: every process rate is computed and
+// then selected by its applicability condition, so that the independent
+// chains (halo scales, ISCO, metallicity, star formation, hot-halo
+// structure) can be interleaved by the scheduler.  A rate that does not
+// apply is exactly zero, which leaves the accumulated sums unchanged.
 __device__ __forceinline__ void rhs(const double* __restrict__ P, const DerivedConstants& D,
                                     const double* __restrict__ tab, const CoolingDesc& cd, int mode,
                                     const NodePar& np, double t, const double (&y)[kNV], double (&f)[kNV]) {
@@ -174,39 +208,59 @@ __device__ __forceinline__ void rhs(const double* __restrict__ P, const DerivedC
     f[2] = -1.0 * y[1];
     return;
   }
+  const bool box = mode == GAL_PHYSICS_BOX;
+  // ---- chains that do not depend on the halo scales ---------------------
+  const double m_bh = y[GAL_V_BH_MASS];
+  const double spin = fmin(fmax(y[GAL_V_BH_SPIN], 0.0), 0.998);
+  double e_isco, l_isco;
+  isco(spin, e_isco, l_isco);
+  const double m_hot = y[GAL_V_HOT_MASS], r_out = y[GAL_V_HOT_OUTER_RADIUS];
+  const bool hot_ok = m_hot > 0.0 && r_out > 0.0;
+  const double inv_m_hot = hot_ok ? rcp(m_hot) : 0.0;
+  const double zf = fmin(fmax(y[GAL_V_HOT_ABUNDANCES] * inv_m_hot, 0.0), 1.0);
+  const double lz = log(clamp_metallicity(cd, zf * (1.0 / kMetallicitySolar)));
+  const double mg_d = y[GAL_V_DISK_MASS_GAS], ms_d = y[GAL_V_DISK_MASS_STELLAR];
+  const double j_d = y[GAL_V_DISK_ANGULAR_MOMENTUM], zg_d = y[GAL_V_DISK_ABUNDANCES_GAS];
+  const double mg_s = y[GAL_V_SPHEROID_MASS_GAS], ms_s = y[GAL_V_SPHEROID_MASS_STELLAR];
+  const double j_s = y[GAL_V_SPHEROID_ANGULAR_MOMENTUM], zg_s = y[GAL_V_SPHEROID_ABUNDANCES_GAS];
+  const double inv_mg_d = mg_d > 0.0 ? rcp(mg_d) : 0.0;
+  const double inv_mg_s = mg_s > 0.0 ? rcp(mg_s) : 0.0;
+  const double mt_d = mg_d + ms_d, mt_s = mg_s + ms_s;
+  const double inv_mt_d = mt_d > 0.0 ? rcp(mt_d) : 0.0;
+  const double inv_mt_s = mt_s > 0.0 ? rcp(mt_s) : 0.0;
+
+  // ---- halo analytics ----------------------------------------------------
   const Halo H = halo(P, D, np, t, y[GAL_V_SPIN_ANGULAR_MOMENTUM]);
   const double v_gal = H.v_vir, r_gal = H.r_gal;
-  const double sqrt_v = sqrt(v_gal);
-  const double inv_sqrt_v = 1.0 / sqrt_v;
+  const double inv_sqrt_v = rsqrt(v_gal);
+  const double sqrt_v = v_gal * inv_sqrt_v;
+  const bool generic_pow = D.n2_sf_disk == kNoHalfPower || D.n2_sf_sph == kNoHalfPower ||
+                           D.n2_fb_disk == kNoHalfPower || D.n2_fb_sph == kNoHalfPower;
+  const double ln_v = generic_pow ? log(v_gal) : 0.0;
+  const double tdyn_gal = kMpcPerKmsToGyr * r_gal * H.inv_v;
+  const double inv_tdyn_gal = H.v_vir * rcp(r_gal) * (1.0 / kMpcPerKmsToGyr);
+  const double t_vir = D.t_vir_per_v2 * H.v_vir * H.v_vir;
+  const double lt = log(clamp_temperature(cd, t_vir));
 
   // spin%angularMomentum growth (halo_angular_momentum_interpolate.F90)
   f[GAL_V_SPIN_ANGULAR_MOMENTUM] += np.satellite ? 0.0 : np.spin_rate;
 
   const double R = P[GAL_P_RECYCLED_FRACTION], yld = P[GAL_P_METAL_YIELD];
-  const double tdyn_gal = kMpcPerKmsToGyr * r_gal * H.inv_v;
-  const bool generic_pow = D.n2_sf_disk == kNoHalfPower || D.n2_sf_sph == kNoHalfPower ||
-                           D.n2_fb_disk == kNoHalfPower || D.n2_fb_sph == kNoHalfPower;
-  const double ln_v = generic_pow ? log(v_gal) : 0.0;
-
   // ---- starFormationDisks (star_formation/disks.F90:200-325) -----------
-  const double mg_d = y[GAL_V_DISK_MASS_GAS], ms_d = y[GAL_V_DISK_MASS_STELLAR];
-  const double j_d = y[GAL_V_DISK_ANGULAR_MOMENTUM], zg_d = y[GAL_V_DISK_ABUNDANCES_GAS];
+  // dynamicalTimeTimescale (star_formation/timescales/dynamical_time.F90:164-188),
+  // timescaleRate (star_formation/rates/disks/timescale.F90:109-130),
+  // instantaneousRates (stellar_populations/properties/instantaneous.F90:133-182)
   const bool disk_ok = !(j_d < 0.0 || r_gal < 0.0 || mg_d < 0.0);
-  const double inv_mg_d = mg_d > 0.0 ? 1.0 / mg_d : 0.0;
-  double sfr_d = 0.0;
-  if (disk_ok) {
-    double tau;
-    if (mode == GAL_PHYSICS_BOX) {
-      tau = P[GAL_P_BOX_TIMESCALE];  // starFormationTimescale fixed
-    } else {
-      // dynamicalTimeTimescale (star_formation/timescales/dynamical_time.F90:164-188)
-      const double eff = P[GAL_P_SF_DISK_EFFICIENCY];
-      const double pw = pow_param(v_gal * D.inv_200, sqrt_v * D.inv_sqrt_200, ln_v - D.ln_200, D.n2_sf_disk,
-                                  P[GAL_P_SF_DISK_EXPONENT_VELOCITY]);
-      tau = (v_gal <= 0.0 || eff == 0.0) ? 0.0 : fmax(tdyn_gal * pw / eff, P[GAL_P_SF_DISK_TIMESCALE_MINIMUM]);
-    }
-    sfr_d = tau > 0.0 ? mg_d / tau : 0.0;  // star_formation/rates/disks/timescale.F90:109-130
-    // instantaneousRates (stellar_populations/properties/instantaneous.F90:133-182)
+  const double x_sf = v_gal * D.inv_200, sqrt_x_sf = sqrt_v * D.inv_sqrt_200;
+  double tau_d;
+  {
+    const double eff = P[GAL_P_SF_DISK_EFFICIENCY];
+    const double pw = pow_param(x_sf, sqrt_x_sf, ln_v - D.ln_200, D.n2_sf_disk, P[GAL_P_SF_DISK_EXPONENT_VELOCITY]);
+    tau_d = (v_gal <= 0.0 || eff == 0.0) ? 0.0 : fmax(tdyn_gal * pw * D.inv_eff_sf_disk, P[GAL_P_SF_DISK_TIMESCALE_MINIMUM]);
+    if (box) tau_d = P[GAL_P_BOX_TIMESCALE];  // starFormationTimescale fixed
+  }
+  const double sfr_d = (disk_ok && tau_d > 0.0) ? mg_d * rcp(tau_d) : 0.0;
+  {
     const double zfuel = zg_d * inv_mg_d;
     const double rs = (1.0 - R) * sfr_d;
     f[GAL_V_DISK_MASS_STELLAR] += rs;
@@ -216,18 +270,15 @@ __device__ __forceinline__ void rhs(const double* __restrict__ P, const DerivedC
     f[GAL_V_DISK_MASS_STELLAR_FORMED] += sfr_d;
   }
   // ---- starFormationSpheroids (star_formation/spheroids.F90:152) --------
-  const double mg_s = y[GAL_V_SPHEROID_MASS_GAS], ms_s = y[GAL_V_SPHEROID_MASS_STELLAR];
-  const double j_s = y[GAL_V_SPHEROID_ANGULAR_MOMENTUM], zg_s = y[GAL_V_SPHEROID_ABUNDANCES_GAS];
-  const bool sph_ok = mode == GAL_PHYSICS_STANDARD && !(j_s < 0.0 || r_gal < 0.0 || mg_s < 0.0);
-  const double inv_mg_s = mg_s > 0.0 ? 1.0 / mg_s : 0.0;
-  double sfr_s = 0.0;
-  if (sph_ok) {
+  const bool sph_ok = !box && !(j_s < 0.0 || r_gal < 0.0 || mg_s < 0.0);
+  double tau_s;
+  {
     const double eff = P[GAL_P_SF_SPHEROID_EFFICIENCY];
-    const double pw = pow_param(v_gal * D.inv_200, sqrt_v * D.inv_sqrt_200, ln_v - D.ln_200, D.n2_sf_sph,
-                                P[GAL_P_SF_SPHEROID_EXPONENT_VELOCITY]);
-    const double tau =
-        (v_gal <= 0.0 || eff == 0.0) ? 0.0 : fmax(tdyn_gal * pw / eff, P[GAL_P_SF_SPHEROID_TIMESCALE_MINIMUM]);
-    sfr_s = tau > 0.0 ? mg_s / tau : 0.0;
+    const double pw = pow_param(x_sf, sqrt_x_sf, ln_v - D.ln_200, D.n2_sf_sph, P[GAL_P_SF_SPHEROID_EXPONENT_VELOCITY]);
+    tau_s = (v_gal <= 0.0 || eff == 0.0) ? 0.0 : fmax(tdyn_gal * pw * D.inv_eff_sf_sph, P[GAL_P_SF_SPHEROID_TIMESCALE_MINIMUM]);
+  }
+  const double sfr_s = (sph_ok && tau_s > 0.0) ? mg_s * rcp(tau_s) : 0.0;
+  {
     const double zfuel = zg_s * inv_mg_s;
     const double rs = (1.0 - R) * sfr_s;
     f[GAL_V_SPHEROID_MASS_STELLAR] += rs;
@@ -236,247 +287,223 @@ __device__ __forceinline__ void rhs(const double* __restrict__ P, const DerivedC
     f[GAL_V_SPHEROID_ABUNDANCES_GAS] += -(zfuel * rs) + yld * sfr_s;
     f[GAL_V_SPHEROID_MASS_STELLAR_FORMED] += sfr_s;
   }
-  // ---- stellarFeedbackDisks (stellar_feedback/disks.F90:116-206) --------
-  // powerLaw (outflows/power_law/_class.F90:118-164) limited by rateLimit
-  // (outflows/rate_limit.F90:111-170); outflowing gas joins the outflowed
-  // reservoir (hot_halo/standard/_class.F90:598-626, fractionStripped=0).
-  const double tdyn_fb = (v_gal <= 0.0 || r_gal <= 0.0) ? 1.0 : tdyn_gal;
-  if (disk_ok) {
-    double out;
-    if (mode == GAL_PHYSICS_BOX) {
-      out = P[GAL_P_BOX_OUTFLOW_FRACTION] * sfr_d;
-    } else {
-      const double w = D.v_fb_disk * H.inv_v;
-      out = v_gal <= 0.0 ? 0.0
-                         : pow_param(w, D.sqrt_v_fb_disk * inv_sqrt_v, D.ln_v_fb_disk - ln_v, D.n2_fb_disk,
-                                     P[GAL_P_FB_DISK_EXPONENT]) * sfr_d;
-    }
-    const double omax = fmax(mg_d / tdyn_fb / P[GAL_P_FB_DISK_TIMESCALE_MINIMUM], 0.0);
-    if (out > omax) out = out * omax / out;
-    if (out > 0.0) {
-      const double mtot = mg_d + ms_d;
-      const double rj = mtot > 0.0 ? j_d * (out / mtot) : 0.0;
-      const double rz = mg_d > 0.0 ? (zg_d * inv_mg_d) * out : 0.0;
-      f[GAL_V_HOT_OUTFLOWED_MASS] += out;
-      f[GAL_V_DISK_MASS_GAS] -= out;
-      f[GAL_V_HOT_OUTFLOWED_ANGULAR_MOMENTUM] += rj;
-      f[GAL_V_DISK_ANGULAR_MOMENTUM] -= rj;
-      f[GAL_V_HOT_OUTFLOWED_ABUNDANCES] += rz;
-      f[GAL_V_DISK_ABUNDANCES_GAS] -= rz;
-    }
+  // ---- stellarFeedbackDisks / Spheroids (stellar_feedback/disks.F90:116-206,
+  // spheroids.F90:116): powerLaw (outflows/power_law/_class.F90:118-164)
+  // limited by rateLimit (outflows/rate_limit.F90:111-170); the outflow joins
+  // the outflowed reservoir (hot_halo/standard/_class.F90:598-626,
+  // fractionStripped = 0).
+  const double inv_tdyn_fb = (v_gal <= 0.0 || r_gal <= 0.0) ? 1.0 : inv_tdyn_gal;
+  double out_d, out_s;
+  {
+    const double w = D.v_fb_disk * H.inv_v;
+    double o = pow_param(w, D.sqrt_v_fb_disk * inv_sqrt_v, D.ln_v_fb_disk - ln_v, D.n2_fb_disk,
+                         P[GAL_P_FB_DISK_EXPONENT]) * sfr_d;
+    o = v_gal <= 0.0 ? 0.0 : o;
+    if (box) o = P[GAL_P_BOX_OUTFLOW_FRACTION] * sfr_d;  // stellarFeedbackOutflows fixed
+    const double omax = fmax(mg_d * inv_tdyn_fb * D.inv_tmin_fb_disk, 0.0);
+    o = o > omax ? omax : o;
+    out_d = (disk_ok && o > 0.0) ? o : 0.0;
   }
-  if (mode == GAL_PHYSICS_BOX) return;
-  // ---- stellarFeedbackSpheroids (stellar_feedback/spheroids.F90:116) ----
-  if (sph_ok) {
+  {
     const double w = D.v_fb_sph * H.inv_v;
-    double out = v_gal <= 0.0 ? 0.0
-                              : pow_param(w, D.sqrt_v_fb_sph * inv_sqrt_v, D.ln_v_fb_sph - ln_v, D.n2_fb_sph,
-                                          P[GAL_P_FB_SPHEROID_EXPONENT]) * sfr_s;
-    const double omax = fmax(mg_s / tdyn_fb / P[GAL_P_FB_SPHEROID_TIMESCALE_MINIMUM], 0.0);
-    if (out > omax) out = out * omax / out;
-    if (out > 0.0) {
-      const double mtot = mg_s + ms_s;
-      const double rj = mtot > 0.0 ? j_s * (out / mtot) : 0.0;
-      const double rz = mg_s > 0.0 ? (zg_s * inv_mg_s) * out : 0.0;
-      f[GAL_V_HOT_OUTFLOWED_MASS] += out;
-      f[GAL_V_SPHEROID_MASS_GAS] -= out;
-      f[GAL_V_HOT_OUTFLOWED_ANGULAR_MOMENTUM] += rj;
-      f[GAL_V_SPHEROID_ANGULAR_MOMENTUM] -= rj;
-      f[GAL_V_HOT_OUTFLOWED_ABUNDANCES] += rz;
-      f[GAL_V_SPHEROID_ABUNDANCES_GAS] -= rz;
-    }
+    double o = pow_param(w, D.sqrt_v_fb_sph * inv_sqrt_v, D.ln_v_fb_sph - ln_v, D.n2_fb_sph,
+                         P[GAL_P_FB_SPHEROID_EXPONENT]) * sfr_s;
+    o = v_gal <= 0.0 ? 0.0 : o;
+    const double omax = fmax(mg_s * inv_tdyn_fb * D.inv_tmin_fb_sph, 0.0);
+    o = o > omax ? omax : o;
+    out_s = (sph_ok && o > 0.0) ? o : 0.0;
+  }
+  {
+    const double rj = j_d * (out_d * inv_mt_d);
+    const double rz = (zg_d * inv_mg_d) * out_d;
+    f[GAL_V_HOT_OUTFLOWED_MASS] += out_d;
+    f[GAL_V_DISK_MASS_GAS] -= out_d;
+    f[GAL_V_HOT_OUTFLOWED_ANGULAR_MOMENTUM] += rj;
+    f[GAL_V_DISK_ANGULAR_MOMENTUM] -= rj;
+    f[GAL_V_HOT_OUTFLOWED_ABUNDANCES] += rz;
+    f[GAL_V_DISK_ABUNDANCES_GAS] -= rz;
+  }
+  if (box) return;
+  {
+    const double rj = j_s * (out_s * inv_mt_s);
+    const double rz = (zg_s * inv_mg_s) * out_s;
+    f[GAL_V_HOT_OUTFLOWED_MASS] += out_s;
+    f[GAL_V_SPHEROID_MASS_GAS] -= out_s;
+    f[GAL_V_HOT_OUTFLOWED_ANGULAR_MOMENTUM] += rj;
+    f[GAL_V_SPHEROID_ANGULAR_MOMENTUM] -= rj;
+    f[GAL_V_HOT_OUTFLOWED_ABUNDANCES] += rz;
+    f[GAL_V_SPHEROID_ABUNDANCES_GAS] -= rz;
   }
   // ---- hot-halo structure and cooling radius ----------------------------
   // beta=2/3 profile, core coreRadiusOverVirialRadius*r_vir, truncated at
   // outerRadius (hot_halo/mass_distribution/beta_profile.F90); virial
   // (isothermal) temperature (hot_halo/temperature_profile/virial.F90).
-  const double m_hot = y[GAL_V_HOT_MASS], r_out = y[GAL_V_HOT_OUTER_RADIUS];
-  const double t_vir = D.t_vir_per_v2 * H.v_vir * H.v_vir;
-  const bool hot_ok = m_hot > 0.0 && r_out > 0.0;
-  double r_core = 1.0, rho0 = 0.0, r_cool = 0.0, r_cool_rate = 0.0, r_mean = 1.0, lambda_cool = 0.0;
-  double inv_m_hot = 0.0, nh_per_rho = 0.0, nall_per_nh = 0.0;
-  if (hot_ok) {
-    inv_m_hot = 1.0 / m_hot;
-    r_core = P[GAL_P_CORE_RADIUS_OVER_VIRIAL] * H.r_vir;
-    const double xo = r_out / r_core;
-    const double inv_shape = 1.0 / (xo - atan(xo));
-    rho0 = m_hot * inv_shape / (4.0 * kPi * r_core * r_core * r_core);
-    r_mean = r_core * (0.5 * xo * xo - 0.5 * log1p(xo * xo)) * inv_shape;
-    // composition of metallicity-only abundances, fully ionized H and He
-    const double zf = fmin(fmax(y[GAL_V_HOT_ABUNDANCES] * inv_m_hot, 0.0), 1.0);
-    constexpr double kNorm = 1.0 / (kHydrogenPrimordial + kHeliumPrimordial);
-    const double X = (1.0 - zf) * kHydrogenPrimordial * kNorm;
-    const double Y = (1.0 - zf) * kHeliumPrimordial * kNorm;
-    const double nh = X / kAtomicMassHydrogen, nhe = Y / kAtomicMassHelium;
-    nh_per_rho = X * D.nh_coeff;
-    nall_per_nh = (nh + nhe) / nh + (1.0 + 2.0 * nhe / nh);
-    lambda_cool = cooling_function(tab, cd, t_vir, zf * (1.0 / kMetallicitySolar));
-    // simpleTime (cooling_time/simple.F90:128-181): t_cool = K / rho
-    const double t_avail = H.t_dyn;  // White-Frenk.F90:144-146 (ageFactor = 0)
-    if (!(lambda_cool > 0.0)) {
-      r_cool = 0.0;  // t_cool = 1e10 Gyr everywhere
-    } else {
-      const double K = D.cool_k_pre * t_vir * nall_per_nh / (lambda_cool * nh_per_rho);
-      // simpleRadius (cooling_radius/simple.F90:313-387)
-      const double rho_out = rho0 / (1.0 + xo * xo);
-      if (K / rho_out - t_avail < 0.0) r_cool = r_out;
-      else if (K / rho0 - t_avail > 0.0) r_cool = 0.0;
-      else r_cool = fmin(r_core * sqrt(fmax(rho0 * t_avail / K - 1.0, 0.0)), r_out);
-    }
-    // simpleRadiusGrowthRate (cooling_radius/simple.F90:229-309); the
-    // time-available growth rate is 1 (White-Frenk.F90:161-168)
-    if (r_cool < r_out && r_cool > 0.0) {
-      const double x = r_cool / r_core;
-      const double slope = 2.0 * x * x / (1.0 + x * x);
-      r_cool_rate = slope != 0.0 ? r_cool / t_avail / slope : 0.0;
-    }
-  }
+  const double r_core = P[GAL_P_CORE_RADIUS_OVER_VIRIAL] * H.r_vir;
+  const double inv_r_core = H.inv_r_vir * D.inv_core;
+  const double xo = r_out * inv_r_core;
+  const double inv_shape = rcp(xo - atan(xo));
+  const double rho0 = m_hot * inv_shape * (1.0 / (4.0 * kPi)) * inv_r_core * inv_r_core * inv_r_core;
+  const double r_mean = r_core * (0.5 * xo * xo - 0.5 * log1p(xo * xo)) * inv_shape;
+  // composition of metallicity-only abundances, fully ionized H and He
+  constexpr double kNorm = 1.0 / (kHydrogenPrimordial + kHeliumPrimordial);
+  const double X = (1.0 - zf) * kHydrogenPrimordial * kNorm;
+  const double Yh = (1.0 - zf) * kHeliumPrimordial * kNorm;
+  const double nh_n = X / kAtomicMassHydrogen, nhe_n = Yh / kAtomicMassHelium;
+  const double nh_per_rho = X * D.nh_coeff;
+  const double inv_nh_n = rcp(nh_n);
+  const double nall_per_nh = (nh_n + nhe_n) * inv_nh_n + (1.0 + 2.0 * nhe_n * inv_nh_n);
+  const double lambda_cool = cooling_function_ln(tab, cd, lt, lz);
+  // simpleTime (cooling_time/simple.F90:128-181): t_cool = K / rho;
+  // simpleRadius (cooling_radius/simple.F90:313-387) with t_available = t_dyn
+  // (time_available/White-Frenk.F90:144-146, ageFactor = 0)
+  const double t_avail = H.t_dyn;
+  const double K = D.cool_k_pre * t_vir * nall_per_nh * rcp(lambda_cool * nh_per_rho);
+  const double inv_k = rcp(K);
+  const double rho_out = rho0 * rcp(1.0 + xo * xo);
+  double r_cool = fmin(r_core * sqrt(fmax(rho0 * t_avail * inv_k - 1.0, 0.0)), r_out);
+  r_cool = (K - t_avail * rho0 > 0.0) ? 0.0 : r_cool;
+  r_cool = (K - t_avail * rho_out < 0.0) ? r_out : r_cool;
+  r_cool = (lambda_cool > 0.0 && hot_ok) ? r_cool : 0.0;
+  // simpleRadiusGrowthRate (cooling_radius/simple.F90:229-309); the
+  // time-available growth rate is 1 (White-Frenk.F90:161-168)
+  const double xc = r_cool * inv_r_core;
+  const double slope = 2.0 * xc * xc / (1.0 + xc * xc);
+  double r_cool_rate = r_cool * H.inv_t_dyn * rcp(slope);
+  r_cool_rate = (r_cool < r_out && r_cool > 0.0 && slope != 0.0) ? r_cool_rate : 0.0;
+
   // ---- blackHolesAccretion (black_holes/accretion.F90:113-170) ----------
   // rates: accretion_rates/standard.F90:236-440; Bondi-Hoyle-Lyttleton:
   // accretion/Bondi_Hoyle_Lyttleton.F90:34-78; thin disk:
   // accretion_disks/Shakura_Sunyaev.F90:70-179.
-  double p_jet = 0.0;
-  const double m_bh = y[GAL_V_BH_MASS];
-  if (m_bh > 0.0) {
-    const double spin = fmin(fmax(y[GAL_V_BH_SPIN], 0.0), 0.998);
-    double e_isco, l_isco;
-    isco(spin, e_isco, l_isco);
-    const double eff_thin = 1.0 - e_isco;
-    const double m_edd = D.m_edd_per_mass * m_bh;
-    const double edd_cap = eff_thin > 0.0 ? m_edd / eff_thin : 0.0;
-    constexpr double kBhlUnit = kKilo * kGigaYear / kMegaParsec * 4.0 * kPi;
-    // spheroid mode
-    double mdot_sph = 0.0;
-    {
-      double racc = P[GAL_P_BH_TEMPERATURE_SPHEROID] > 0.0 ? kGravity * m_bh / D.cs2_sph : HUGE_VAL;
-      racc = fmax(racc, y[GAL_V_BH_RADIAL_POSITION]);
-      double rho = hernquist_density(mg_s, r_gal, racc);
-      if (rho > 1.0) {
-        const double lj = fmin(sqrt(D.cs2_sph / (kGravity * rho)), r_gal);
-        if (lj > racc) rho = hernquist_density(mg_s, r_gal, lj);
-        double m = fmax(P[GAL_P_BH_ENHANCEMENT_SPHEROID] * kBhlUnit * sq(kGravity * m_bh) * rho /
-                            (D.cs2_sph * sqrt(D.cs2_sph)),
-                        0.0);
-        if (m > 0.0 && eff_thin > 0.0) m = fmin(m, edd_cap);
-        mdot_sph = m;
-      }
-    }
-    // hot-halo mode (hot mode only)
-    double mdot_hot = 0.0;
-    if (hot_ok) {
-      const double cs2 = D.cs2_per_t * t_vir;
-      const double r2 = fmin(kGravity * m_bh / cs2, r_out);
-      const double q = r_cool / H.r_vir;
-      double fhot;  // standardHotModeFraction (accretion_rates/standard.F90:442-466)
-      if (q < 0.9) fhot = 1.0;
-      else if (q > 1.0) fhot = 0.0;
-      else { const double x = (q - 0.9) / (1.0 - 0.9); fhot = x * x * (2.0 * x - 3.0) + 1.0; }
-      const double rho2 = fhot * (rho0 / (1.0 + sq(r2 / r_core)));
-      if (rho2 > 1.0) {
-        const double cs = sqrt(cs2);
-        double m = fmax(P[GAL_P_BH_ENHANCEMENT_HOT_HALO] * kBhlUnit * r2 * r2 * rho2 * cs, 0.0);
-        const double mmax = fmax(m_hot * cs / (r_out * kMpcPerKmsToGyr), 0.0);
-        m = fmin(m, mmax);
-        if (m > 0.0 && eff_thin > 0.0) m = fmin(m, edd_cap);
-        mdot_hot = m;
-      }
-    }
-    const double mdot = mdot_sph + mdot_hot;
-    if (mdot > 0.0) {
-      // Meier (2001) jet power (Shakura_Sunyaev.F90:92-140)
-      const double lm = -0.3 * log(m_bh) + 1.2 * log(mdot);
-      p_jet = spin > 0.8 ? exp(D.ln_jet_kerr + lm) * (1.0 + 1.1 * spin + 0.29 * spin * spin)
-                         : exp(D.ln_jet_schw + lm + 3.785 * spin);
-      constexpr double kC2 = (kSpeedLight / kKilo) * (kSpeedLight / kKilo);
-      const double eff_jet = p_jet / mdot / kC2;
-      const double reduced = mdot * (1.0 - eff_thin - eff_jet);
-      const double spin_up = (l_isco - 2.0 * spin * e_isco) * mdot / m_bh;
-      f[GAL_V_BH_MASS] += reduced;
-      if (mdot_sph != 0.0) {
-        f[GAL_V_SPHEROID_MASS_GAS] -= mdot_sph;
-        if (mg_s > 0.0) f[GAL_V_SPHEROID_ABUNDANCES_GAS] -= mdot_sph * (zg_s * inv_mg_s);
-        if (mg_s + ms_s > 0.0) f[GAL_V_SPHEROID_ANGULAR_MOMENTUM] -= mdot_sph * (j_s / (mg_s + ms_s));
-      }
-      if (mdot_hot != 0.0 && m_hot > 0.0) {
-        f[GAL_V_HOT_MASS] -= mdot_hot;
-        f[GAL_V_HOT_ABUNDANCES] -= mdot_hot * (y[GAL_V_HOT_ABUNDANCES] * inv_m_hot);
-        f[GAL_V_HOT_ANGULAR_MOMENTUM] -= mdot_hot * (y[GAL_V_HOT_ANGULAR_MOMENTUM] * inv_m_hot);
-      }
-      f[GAL_V_BH_SPIN] += spin_up;
-    }
+  const bool bh_ok = m_bh > 0.0;
+  const double eff_thin = 1.0 - e_isco;
+  const double edd_cap = D.m_edd_per_mass * m_bh * rcp(eff_thin);
+  constexpr double kBhlUnit = kKilo * kGigaYear / kMegaParsec * 4.0 * kPi;
+  double mdot_sph;
+  {
+    double racc = P[GAL_P_BH_TEMPERATURE_SPHEROID] > 0.0 ? kGravity * m_bh * D.inv_cs2_sph : HUGE_VAL;
+    racc = fmax(racc, y[GAL_V_BH_RADIAL_POSITION]);
+    const double rho_a = hernquist_density(mg_s, r_gal, racc);
+    const double lj = fmin(rsqrt(D.jeans_coeff * rho_a), r_gal);
+    const double rho_j = hernquist_density(mg_s, r_gal, lj);
+    const double rho = lj > racc ? rho_j : rho_a;
+    double m = fmax(D.bhl_sph_coeff * m_bh * m_bh * rho, 0.0);
+    m = (m > 0.0 && eff_thin > 0.0) ? fmin(m, edd_cap) : m;
+    mdot_sph = (bh_ok && rho_a > 1.0) ? m : 0.0;
+  }
+  double mdot_hot;
+  {
+    const double cs2 = D.cs2_per_t * t_vir;
+    const double cs = sqrt(cs2);
+    const double r2 = fmin(kGravity * m_bh * rcp(cs2), r_out);
+    const double q = r_cool * H.inv_r_vir;
+    // standardHotModeFraction (accretion_rates/standard.F90:442-466)
+    const double xq = (q - 0.9) * 10.0;
+    double fhot = xq * xq * (2.0 * xq - 3.0) + 1.0;
+    fhot = q < 0.9 ? 1.0 : (q > 1.0 ? 0.0 : fhot);
+    const double rho2 = fhot * (rho0 * rcp(1.0 + sq(r2 * inv_r_core)));
+    double m = fmax(P[GAL_P_BH_ENHANCEMENT_HOT_HALO] * kBhlUnit * r2 * r2 * rho2 * cs, 0.0);
+    m = fmin(m, fmax(m_hot * cs * rcp(r_out) * (1.0 / kMpcPerKmsToGyr), 0.0));
+    m = (m > 0.0 && eff_thin > 0.0) ? fmin(m, edd_cap) : m;
+    mdot_hot = (bh_ok && hot_ok && rho2 > 1.0) ? m : 0.0;
+  }
+  const double mdot = mdot_sph + mdot_hot;
+  const bool acc_ok = bh_ok && mdot > 0.0;
+  // Meier (2001) jet power (Shakura_Sunyaev.F90:92-140):
+  // M^-0.3 Mdot^1.2 = (Mdot^4 / M)^0.3
+  const double inv_m_bh = rcp(m_bh);
+  const double lm = 0.3 * log(mdot * mdot * mdot * mdot * inv_m_bh);
+  const double p_jet_k = exp(D.ln_jet_kerr + lm) * (1.0 + 1.1 * spin + 0.29 * spin * spin);
+  const double p_jet_s = exp(D.ln_jet_schw + lm + 3.785 * spin);
+  const double p_jet = acc_ok ? (spin > 0.8 ? p_jet_k : p_jet_s) : 0.0;
+  {
+    constexpr double kInvC2 = 1.0 / ((kSpeedLight / kKilo) * (kSpeedLight / kKilo));
+    const double eff_jet = p_jet * rcp(mdot) * kInvC2;
+    const double reduced = mdot * (1.0 - eff_thin - eff_jet);
+    const double spin_up = (l_isco - 2.0 * spin * e_isco) * mdot * inv_m_bh;
+    f[GAL_V_BH_MASS] += acc_ok ? reduced : 0.0;
+    // spheroid%massGasSinkRate: gas, metals and angular momentum in proportion
+    f[GAL_V_SPHEROID_MASS_GAS] -= acc_ok ? mdot_sph : 0.0;
+    f[GAL_V_SPHEROID_ABUNDANCES_GAS] -= acc_ok ? mdot_sph * (zg_s * inv_mg_s) : 0.0;
+    f[GAL_V_SPHEROID_ANGULAR_MOMENTUM] -= acc_ok ? mdot_sph * (j_s * inv_mt_s) : 0.0;
+    // hotHalo%massSinkRate: mass, metals and angular momentum in proportion
+    const bool hs = acc_ok && hot_ok;
+    f[GAL_V_HOT_MASS] -= hs ? mdot_hot : 0.0;
+    f[GAL_V_HOT_ABUNDANCES] -= hs ? mdot_hot * (y[GAL_V_HOT_ABUNDANCES] * inv_m_hot) : 0.0;
+    f[GAL_V_HOT_ANGULAR_MOMENTUM] -= hs ? mdot_hot * (y[GAL_V_HOT_ANGULAR_MOMENTUM] * inv_m_hot) : 0.0;
+    f[GAL_V_BH_SPIN] += acc_ok ? spin_up : 0.0;
   }
   // ---- CGMAccretion (circumgalactic_medium/accretion.F90:517) with
   //      accretionHalo simple (accretion/halo/simple.F90:281-378) ---------
-  if (!np.satellite && H.mass > 0.0) {
+  {
+    const bool acc = !np.satellite && H.mass > 0.0;
     const double failed = (H.a3 > D.a3_reion && H.v_vir < P[GAL_P_VELOCITY_SUPPRESSION_REIONIZATION]) ? 1.0 : 0.0;
-    const double growth = np.m_rate / H.mass;
+    const double inv_mass = rcp(H.mass);
+    const double growth = np.m_rate * inv_mass;
     const double m_un = y[GAL_V_HOT_UNACCRETED_MASS], z_un = y[GAL_V_HOT_UNACCRETED_ABUNDANCES];
     const double rate_acc = D.f_baryon * np.m_rate * (1.0 - failed) + m_un * growth * (1.0 - failed);
     const double rate_fail = D.f_baryon * np.m_rate * failed - m_un * growth * (1.0 - failed);
     const double rate_z = z_un * growth * (1.0 - failed);
-    f[GAL_V_HOT_MASS] += rate_acc;
-    f[GAL_V_HOT_UNACCRETED_MASS] += rate_fail;
-    f[GAL_V_HOT_ABUNDANCES] += rate_z;
-    f[GAL_V_HOT_UNACCRETED_ABUNDANCES] -= rate_z;
-    f[GAL_V_HOT_ANGULAR_MOMENTUM] += rate_acc * (y[GAL_V_SPIN_ANGULAR_MOMENTUM] / H.mass);
+    f[GAL_V_HOT_MASS] += acc ? rate_acc : 0.0;
+    f[GAL_V_HOT_UNACCRETED_MASS] += acc ? rate_fail : 0.0;
+    f[GAL_V_HOT_ABUNDANCES] += acc ? rate_z : 0.0;
+    f[GAL_V_HOT_UNACCRETED_ABUNDANCES] -= acc ? rate_z : 0.0;
+    f[GAL_V_HOT_ANGULAR_MOMENTUM] += acc ? rate_acc * (y[GAL_V_SPIN_ANGULAR_MOMENTUM] * inv_mass) : 0.0;
   }
   // ---- CGMOutflowReincorporation (outflow_reincorporation.F90:272-330)
   //      haloDynamicalTimeRate (halo_dynamical_time.F90:113-125) ----------
   {
     const double m_o = y[GAL_V_HOT_OUTFLOWED_MASS];
-    if (m_o > 0.0) {
-      const double r = m_o * P[GAL_P_REINCORPORATION_MULTIPLIER] / H.t_dyn;
-      const double q = r / m_o;
-      const double rj = y[GAL_V_HOT_OUTFLOWED_ANGULAR_MOMENTUM] * q;
-      const double rz = y[GAL_V_HOT_OUTFLOWED_ABUNDANCES] * q;
-      f[GAL_V_HOT_OUTFLOWED_MASS] -= r;
-      f[GAL_V_HOT_OUTFLOWED_ANGULAR_MOMENTUM] -= rj;
-      f[GAL_V_HOT_OUTFLOWED_ABUNDANCES] -= rz;
-      f[GAL_V_HOT_MASS] += r;
-      f[GAL_V_HOT_ANGULAR_MOMENTUM] += rj;
-      f[GAL_V_HOT_ABUNDANCES] += rz;
-    }
+    const bool ok = m_o > 0.0;
+    const double q = P[GAL_P_REINCORPORATION_MULTIPLIER] * H.inv_t_dyn;
+    const double r = m_o * q;
+    const double rr = ok ? r : 0.0;
+    const double rj = ok ? y[GAL_V_HOT_OUTFLOWED_ANGULAR_MOMENTUM] * q : 0.0;
+    const double rz = ok ? y[GAL_V_HOT_OUTFLOWED_ABUNDANCES] * q : 0.0;
+    f[GAL_V_HOT_OUTFLOWED_MASS] -= rr;
+    f[GAL_V_HOT_OUTFLOWED_ANGULAR_MOMENTUM] -= rj;
+    f[GAL_V_HOT_OUTFLOWED_ABUNDANCES] -= rz;
+    f[GAL_V_HOT_MASS] += rr;
+    f[GAL_V_HOT_ANGULAR_MOMENTUM] += rj;
+    f[GAL_V_HOT_ABUNDANCES] += rz;
   }
   // ---- CGMCoolingHeating (circumgalactic_medium/cooling_heating.F90:216-381)
   //      whiteFrenk1991Rate (cooling_rate/White-Frenk.F90:132-185) ---------
-  if (hot_ok && y[GAL_V_HOT_ANGULAR_MOMENTUM] > 0.0) {
-    double cool;
-    if (H.v_vir > P[GAL_P_COOLING_VELOCITY_CUTOFF]) cool = 0.0;
-    else if (r_cool >= r_out) cool = m_hot / H.t_dyn;
-    else cool = 4.0 * kPi * r_cool * r_cool * (rho0 / (1.0 + sq(r_cool / r_core))) * r_cool_rate;
+  {
+    const bool ok = hot_ok && y[GAL_V_HOT_ANGULAR_MOMENTUM] > 0.0;
+    double cool = 4.0 * kPi * r_cool * r_cool * (rho0 * rcp(1.0 + xc * xc)) * r_cool_rate;
+    cool = r_cool >= r_out ? m_hot * H.inv_t_dyn : cool;
+    cool = H.v_vir > P[GAL_P_COOLING_VELOCITY_CUTOFF] ? 0.0 : cool;
     const double heat = P[GAL_P_EFFICIENCY_RADIO_MODE] * p_jet * sq(H.inv_v);
-    if (heat > cool) {
-      const double out = fmin(heat - cool, P[GAL_P_RATE_MAXIMUM_EXPULSION] * m_hot / H.t_dyn);
-      const double q = out * inv_m_hot;
-      const double rz = y[GAL_V_HOT_ABUNDANCES] * q;
-      const double rj = y[GAL_V_HOT_ANGULAR_MOMENTUM] * q;
-      f[GAL_V_HOT_MASS] -= out;
-      f[GAL_V_HOT_ABUNDANCES] -= rz;
-      f[GAL_V_HOT_ANGULAR_MOMENTUM] -= rj;
-      // hotHaloMassEjection: the ejected gas joins the outflowed reservoir
-      f[GAL_V_HOT_OUTFLOWED_MASS] += out;
-      f[GAL_V_HOT_OUTFLOWED_ABUNDANCES] += rz;
-      f[GAL_V_HOT_OUTFLOWED_ANGULAR_MOMENTUM] += rj;
-    } else if (cool > heat) {
-      cool = fmax(cool - heat, 0.0);
-      // constantRotation specific angular momentum: j(r) = (J/M) r / <r>
-      const double rj = cool * ((y[GAL_V_HOT_ANGULAR_MOMENTUM] * inv_m_hot) * r_cool / r_mean);
-      const double rz = cool * y[GAL_V_HOT_ABUNDANCES] * inv_m_hot;
-      f[GAL_V_HOT_MASS] -= cool;
-      f[GAL_V_HOT_ANGULAR_MOMENTUM] -= rj;
-      f[GAL_V_HOT_ABUNDANCES] -= rz;
-      f[GAL_V_DISK_MASS_GAS] += cool;
-      f[GAL_V_DISK_ABUNDANCES_GAS] += rz;
-      f[GAL_V_DISK_ANGULAR_MOMENTUM] += rj * (1.0 - P[GAL_P_FRACTION_LOSS_ANGULAR_MOMENTUM]);
-    }
+    const bool heating = ok && heat > cool;
+    const bool cooling = ok && cool > heat;
+    // net heating expels gas into the outflowed reservoir (hotHaloMassEjection)
+    const double out = heating ? fmin(heat - cool, P[GAL_P_RATE_MAXIMUM_EXPULSION] * m_hot * H.inv_t_dyn) : 0.0;
+    // net cooling onto the disk; constantRotation j(r) = (J/M) r / <r>
+    const double c = cooling ? fmax(cool - heat, 0.0) : 0.0;
+    const double j_hot = y[GAL_V_HOT_ANGULAR_MOMENTUM] * inv_m_hot;
+    const double z_hot = y[GAL_V_HOT_ABUNDANCES] * inv_m_hot;
+    const double oz = y[GAL_V_HOT_ABUNDANCES] * (out * inv_m_hot);
+    const double oj = y[GAL_V_HOT_ANGULAR_MOMENTUM] * (out * inv_m_hot);
+    const double cj = c * (j_hot * r_cool * rcp(r_mean));
+    const double cz = c * z_hot;
+    f[GAL_V_HOT_MASS] -= heating ? out : c;
+    f[GAL_V_HOT_ABUNDANCES] -= heating ? oz : cz;
+    f[GAL_V_HOT_ANGULAR_MOMENTUM] -= heating ? oj : cj;
+    f[GAL_V_HOT_OUTFLOWED_MASS] += out;
+    f[GAL_V_HOT_OUTFLOWED_ABUNDANCES] += heating ? oz : 0.0;
+    f[GAL_V_HOT_OUTFLOWED_ANGULAR_MOMENTUM] += heating ? oj : 0.0;
+    f[GAL_V_DISK_MASS_GAS] += c;
+    f[GAL_V_DISK_ABUNDANCES_GAS] += cz;
+    f[GAL_V_DISK_ANGULAR_MOMENTUM] += cj * (1.0 - P[GAL_P_FRACTION_LOSS_ANGULAR_MOMENTUM]);
   }
   // ---- coolingEnergyRadiated (cooling_energy_radiated.F90:200-260) -------
-  if (hot_ok) {
-    const double rho = m_hot / (4.0 / 3.0 * kPi * r_out * r_out * r_out);
+  {
+    const double rho = m_hot * (3.0 / (4.0 * kPi)) * rcp(r_out * r_out * r_out);
     const double nh = rho * nh_per_rho;
     const double rcm = r_out * (kMegaParsec / kCenti);
     const double count = nh * nall_per_nh * (4.0 * kPi / 3.0) * rcm * rcm * rcm;
-    f[GAL_V_HOT_ENERGY_RADIATED] += lambda_cool * nh * count;
+    f[GAL_V_HOT_ENERGY_RADIATED] += hot_ok ? lambda_cool * nh * count : 0.0;
   }
 }
 

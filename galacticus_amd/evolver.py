@@ -57,7 +57,8 @@ class NodeEvolverBatched:
                  odeToleranceAbsolute: float = gp.EVOLVER_DEFAULTS["odeToleranceAbsolute"],
                  odeToleranceRelative: float = gp.EVOLVER_DEFAULTS["odeToleranceRelative"],
                  reuseODEStepSize: bool = True, enforceNonNegativity: bool = False,
-                 scales_provided: bool = False, odeAlgorithm: str = "rungeKuttaCashKarp"):
+                 scales_provided: bool = False, odeAlgorithm: str = "rungeKuttaCashKarp",
+                 cost_order: bool = True):
         if odeAlgorithm != "rungeKuttaCashKarp":
             raise ValueError("odeAlgorithm must be rungeKuttaCashKarp for the batched evolver")
         self.L = _lib.load()
@@ -65,7 +66,8 @@ class NodeEvolverBatched:
         self.odeToleranceAbsolute = float(odeToleranceAbsolute)
         self.odeToleranceRelative = float(odeToleranceRelative)
         self.flags = gp.solver_flags(dict(reuseODEStepSize=reuseODEStepSize,
-                                          enforceNonNegativity=enforceNonNegativity), scales_provided)
+                                          enforceNonNegativity=enforceNonNegativity), scales_provided,
+                                        cost_order)
         h = ctypes.c_void_p()
         _lib.check(self.L.gal_ode_batch_create(self.n_max, self.odeToleranceAbsolute, self.odeToleranceRelative,
                                                self.flags, ctypes.byref(h)), "gal_ode_batch_create")

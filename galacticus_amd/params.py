@@ -132,6 +132,7 @@ EVOLVER_DEFAULTS = {
 FLAG_ENFORCE_NON_NEGATIVITY = 0x1
 FLAG_REUSE_STEP_SIZE = 0x2
 FLAG_SCALES_PROVIDED = 0x4
+FLAG_COST_ORDER = 0x8
 
 
 def physics_params(**overrides) -> np.ndarray:
@@ -240,8 +241,8 @@ def parse_parameter_xml(path_or_text: str):
     return physics_params(**overrides), evolver
 
 
-def solver_flags(evolver: dict, scales_provided: bool = False) -> int:
-    f = 0
+def solver_flags(evolver: dict, scales_provided: bool = False, cost_order: bool = True) -> int:
+    f = FLAG_COST_ORDER if cost_order else 0
     if evolver.get("enforceNonNegativity"):
         f |= FLAG_ENFORCE_NON_NEGATIVITY
     if evolver.get("reuseODEStepSize", True):
