@@ -15,7 +15,7 @@ LIB_PATH = os.path.join(HERE, "lib", "libgalacticus_amd.so")
 GAL_OK = 0
 ERRORS = {-1: "GAL_E_INVALID", -2: "GAL_E_HIP", -3: "GAL_E_NOMEM", -4: "GAL_E_NODEVICE"}
 
-# Symbols declared in include/gal_ode.h and include/gal_surrogate.h.
+# Symbols declared in include/gal_ode.h.
 EXPORTS = [
     "gal_ode_version", "gal_ode_device_check", "gal_ode_batch_create", "gal_ode_batch_destroy",
     "gal_ode_batch_set_physics", "gal_ode_batch_set_cooling_table", "gal_ode_batch_set_launch",
@@ -66,25 +66,37 @@ def load():
 
 
 SURROGATE_EXPORTS = [
-    "gal_mlp_create", "gal_mlp_destroy", "gal_mlp_set_weights", "gal_mlp_get_weights", "gal_mlp_forward_d",
-    "gal_mlp_train_step_d", "gal_mlp_grad_d", "gal_surrogate_step_d", "gal_mlp_param_count",
+    "gal_mlp_param_count", "gal_mlp_create", "gal_mlp_destroy", "gal_mlp_set_params", "gal_mlp_get_params",
+    "gal_mlp_set_adam", "gal_mlp_set_loss", "gal_mlp_forward_d", "gal_mlp_gradient_d", "gal_mlp_set_dataset",
+    "gal_mlp_train_d", "gal_surrogate_encode_d", "gal_surrogate_apply_d", "gal_surrogate_evolve_d",
 ]
 
 
+class Gate(ctypes.Structure):
+    """gal_surrogate_gate of include/gal_surrogate.h."""
+    _fields_ = [("log_error_max", ctypes.c_double), ("mass_tolerance", ctypes.c_double),
+                ("metal_tolerance", ctypes.c_double), ("bh_slack", ctypes.c_double)]
+
+
 def _load_surrogate(L):
-    vp, c_int, c_float, c_double = ctypes.c_void_p, ctypes.c_int, ctypes.c_float, ctypes.c_double
-    if not hasattr(L, "gal_mlp_create"):
-        return
+    vp, c_int, c_double, i64 = ctypes.c_void_p, ctypes.c_int, ctypes.c_double, ctypes.c_int64
+    fp, ip = ctypes.POINTER(ctypes.c_float), ctypes.POINTER(ctypes.c_int)
     L.gal_mlp_param_count.argtypes = []
     L.gal_mlp_create.argtypes = [c_int, ctypes.POINTER(vp)]
     L.gal_mlp_destroy.argtypes = [vp]
-    L.gal_mlp_set_weights.argtypes = [vp, vp]
-    L.gal_mlp_get_weights.argtypes = [vp, vp]
-    L.gal_mlp_forward_d.argtypes = [vp, c_int, vp, vp, vp]
-    L.gal_mlp_grad_d.argtypes = [vp, c_int, vp, vp, vp, vp, vp]
-    L.gal_mlp_train_step_d.argtypes = [vp, c_int, vp, vp, vp, c_float, c_float, c_float, c_float, c_float, c_int,
-                                       vp, vp]
-    L.gal_surrogate_step_d.argtypes = [vp, vp, c_int, c_int, vp, vp, vp, vp, vp, vp, vp, c_double, c_double, vp,
-                                       vp, vp]
+    L.gal_mlp_set_params.argtypes = [vp, fp, i64]
+    L.gal_mlp_get_params.argtypes = [vp, fp, i64]
+    L.gal_mlp_set_adam.argtypes = [vp, c_double, c_double, c_double, c_double, c_double, i64]
+    L.gal_mlp_set_loss.argtypes = [vp, fp, fp, ip, c_double]
+    L.gal_mlp_forward_d.argtypes = [vp, i64, vp, vp, vp]
+    L.gal_mlp_gradient_d.argtypes = [vp, c_int, vp, vp, vp, vp, vp]
+    L.gal_mlp_set_dataset.argtypes = [vp, i64, vp, vp, vp]
+    L.gal_mlp_train_d.argtypes = [vp, c_int, c_int, vp, vp]
+    L.gal_surrogate_encode_d.argtypes = [vp, c_int, c_int, vp, vp, vp, vp, vp, vp, vp, vp, vp, vp]
+    L.gal_surrogate_apply_d.argtypes = [vp, ctypes.POINTER(Gate), c_int, c_int, vp, vp, vp, vp, vp, vp, vp, vp, vp,
+                                        vp, vp, vp]
+    L.gal_surrogate_evolve_d.argtypes = [vp, vp, vp, ctypes.POINTER(Gate), c_int, c_int, vp, vp, vp, vp, vp, vp,
+                                         vp, vp, ip, vp]
     for name in SURROGATE_EXPORTS:
         getattr(L, name).restype = c_int
+    L.gal_mlp_param_count.restype = i64

@@ -43,6 +43,7 @@
 #include "constants.hpp"
 #include "physics.hpp"
 #include "integrator.hpp"
+#include "batch.hpp"
 
 #include <hipcub/device/device_radix_sort.hpp>
 
@@ -58,6 +59,7 @@ constexpr int kLdsVectors = kStages + 1 + GAL_SC_IN_LDS;
 
 struct EvolveArgs {
   int n, ld;
+  int n_claim;  // claims are limited to n_claim entries of order[] (n when order is null)
   double* y;
   const double* np;
   const double* t_start;
@@ -75,22 +77,6 @@ struct EvolveArgs {
   double eps_abs, eps_rel;
   int flags, mode;
 };
-
-__device__ __forceinline__ NodePar load_node_par(const double* __restrict__ np, int ld, int i) {
-  NodePar p;
-  p.t_ref = np[GAL_NP_TIME_REFERENCE * (size_t)ld + i];
-  p.m_ref = np[GAL_NP_MASS_REFERENCE * (size_t)ld + i];
-  p.m_rate = np[GAL_NP_MASS_RATE * (size_t)ld + i];
-  p.spin_rate = np[GAL_NP_SPIN_RATE * (size_t)ld + i];
-  p.satellite = np[GAL_NP_IS_SATELLITE * (size_t)ld + i] != 0.0;
-  return p;
-}
-
-// isNonNegative attributes of the state variables (component _class.F90 XML).
-__device__ __forceinline__ bool var_non_negative(int v) {
-  return !(v == GAL_V_HOT_ANGULAR_MOMENTUM || v == GAL_V_HOT_OUTFLOWED_ANGULAR_MOMENTUM ||
-           v == GAL_V_HOT_ENERGY_RADIATED || v == GAL_V_BH_RADIAL_POSITION || v == GAL_V_SPIN_ANGULAR_MOMENTUM);
-}
 
 // Claim one node index per requesting lane with a single atomic per wave.
 __device__ __forceinline__ int wave_claim(bool want, unsigned int* work, int n, const int* order) {
@@ -154,7 +140,7 @@ __global__ __launch_bounds__(BLOCK, 1) void evolve_kernel(EvolveArgs a) {
     if (phase == 5) {
       // ---- claim nodes for lanes that need one (convergent point) ------
       while (__any(want)) {
-        const int idx = wave_claim(want, a.work, a.n, a.order);
+        const int idx = wave_claim(want, a.work, a.n_claim, a.order);
         if (want) {
           if (idx < 0) {
             want = false;
@@ -413,13 +399,15 @@ __global__ void hadjust_kernel(int n, int dim, int ord, double eps_abs, double e
 // start point; cost key = |t_end - t_start| / sqrt(tau), tau = the shortest
 // e-folding time min_v (|y_v| + scale_v) / |dy_v/dt|.  Only the order in
 // which lanes claim nodes depends on it, never a result.
-__global__ void cost_probe_kernel(int n, int ld, const double* __restrict__ y, const double* __restrict__ npa,
-                                  const double* __restrict__ t_start, const double* __restrict__ t_end,
-                                  const double* __restrict__ scale_in, const double* __restrict__ P,
-                                  const double* __restrict__ tab, DerivedConstants D, CoolingDesc cd, int mode,
-                                  int scales_given, float* __restrict__ key, int* __restrict__ index) {
-  const int i = blockIdx.x * blockDim.x + threadIdx.x;
-  if (i >= n) return;
+__global__ void cost_probe_kernel(int n_items, const int* __restrict__ subset, int ld, const double* __restrict__ y,
+                                  const double* __restrict__ npa, const double* __restrict__ t_start,
+                                  const double* __restrict__ t_end, const double* __restrict__ scale_in,
+                                  const double* __restrict__ P, const double* __restrict__ tab, DerivedConstants D,
+                                  CoolingDesc cd, int mode, int scales_given, float* __restrict__ key,
+                                  int* __restrict__ index) {
+  const int j = blockIdx.x * blockDim.x + threadIdx.x;
+  if (j >= n_items) return;
+  const int i = subset ? subset[j] : j;
   double yy[kNV], f[kNV], sc[kNV];
 #pragma unroll
   for (int v = 0; v < kNV; ++v) yy[v] = y[v * (size_t)ld + i];
@@ -443,8 +431,8 @@ __global__ void cost_probe_kernel(int n, int ld, const double* __restrict__ y, c
     tau = (r > 0.0 && tv < tau) ? tv : tau;
   }
   const double k = fabs(t_end[i] - t0) * rsqrt(fmax(tau, 1.0e-30));
-  key[i] = (float)fmin(k, 3.0e38);
-  index[i] = i;
+  key[j] = (float)fmin(k, 3.0e38);
+  index[j] = i;
 }
 
 template <int BLOCK>
@@ -457,42 +445,6 @@ constexpr size_t lds_bytes() {
 // ============================================================================
 // C ABI
 // ============================================================================
-struct gal_ode_batch {
-  int n_max = 0;
-  double tol_abs = 0.0, tol_rel = 0.0;
-  int flags = 0;
-  int device = 0;
-  int n_cus = 0;
-  int nodes_per_block = 64;
-  int blocks_per_cu = 1;
-  double params[GAL_NPARAM] = {};
-  DerivedConstants D{};
-  CoolingDesc cd{};
-  double* d_params = nullptr;
-  double* d_table = nullptr;
-  unsigned int* d_work = nullptr;
-  int* d_order = nullptr;
-  int* d_index = nullptr;
-  float* d_key = nullptr;
-  float* d_key_sorted = nullptr;
-  void* d_sort_temp = nullptr;
-  size_t sort_temp_bytes = 0;
-  bool have_physics = false, have_table = false;
-  // staging for the host-pointer entry point
-  double *d_y = nullptr, *d_np = nullptr, *d_t0 = nullptr, *d_t1 = nullptr, *d_h = nullptr, *d_scale = nullptr;
-  int* d_status = nullptr;
-  long long* d_counters = nullptr;
-};
-
-#define HIP_TRY(x)                                                              \
-  do {                                                                          \
-    hipError_t e_ = (x);                                                        \
-    if (e_ != hipSuccess) {                                                     \
-      fprintf(stderr, "gal_ode: %s failed: %s\n", #x, hipGetErrorString(e_));   \
-      return GAL_E_HIP;                                                         \
-    }                                                                           \
-  } while (0)
-
 static int half_power_code(double alpha) {
   const double a2 = 2.0 * alpha;
   if (a2 == std::floor(a2) && std::fabs(a2) <= 16.0) return (int)a2;
@@ -707,6 +659,7 @@ static int make_args(gal_ode_batch* h, int n, int ld, double* y, const double* n
   if (n > 0 && (!y || !node_par || !t_start || !t_end || !h_step || !status)) return GAL_E_INVALID;
   if ((h->flags & GAL_FLAG_SCALES_PROVIDED) && !scale) return GAL_E_INVALID;
   a.n = n;
+  a.n_claim = n;
   a.ld = ld;
   a.y = y;
   a.np = node_par;
@@ -732,23 +685,8 @@ static int make_args(gal_ode_batch* h, int n, int ld, double* y, const double* n
 int gal_ode_batch_evolve_d(gal_ode_batch* h, int n, int ld, double* y, const double* node_par,
                            const double* t_start, const double* t_end, double* h_step, double* scale, int* status,
                            int64_t* counters, void* stream) {
-  EvolveArgs a;
-  int rc = make_args(h, n, ld, y, node_par, t_start, t_end, h_step, scale, status, counters, a);
-  if (rc != GAL_OK) return rc;
-  if (n == 0) return GAL_OK;
-  hipStream_t s = (hipStream_t)stream;
-  if (h->flags & GAL_FLAG_COST_ORDER) {
-    const int threads = 256;
-    hipLaunchKernelGGL(cost_probe_kernel, dim3((n + threads - 1) / threads), dim3(threads), 0, s, n, ld, y, node_par,
-                       t_start, t_end, (const double*)scale, (const double*)h->d_params, (const double*)h->d_table,
-                       h->D, h->cd, a.mode, (h->flags & GAL_FLAG_SCALES_PROVIDED) ? 1 : 0, h->d_key, h->d_index);
-    HIP_TRY(hipGetLastError());
-    size_t tb = h->sort_temp_bytes;
-    HIP_TRY(hipcub::DeviceRadixSort::SortPairsDescending(h->d_sort_temp, tb, h->d_key, h->d_key_sorted, h->d_index,
-                                                         h->d_order, n, 0, 32, s));
-    a.order = h->d_order;
-  }
-  return launch_evolve(h, a, s);
+  return gal_ode_evolve_subset(h, n, ld, y, node_par, t_start, t_end, h_step, scale, status, counters, nullptr, n,
+                               (hipStream_t)stream);
 }
 
 int gal_ode_batch_evolve_h(gal_ode_batch* h, int n, double* y, const double* node_par, const double* t_start,
@@ -871,3 +809,28 @@ int gal_ode_batch_autotune(gal_ode_batch* h, int n, int ld, double* y, const dou
 }
 
 }  // extern "C"
+
+int gal_ode_evolve_subset(gal_ode_batch* h, int n, int ld, double* y, const double* node_par, const double* t_start,
+                          const double* t_end, double* h_step, double* scale, int* status, int64_t* counters,
+                          const int* subset, int n_subset, hipStream_t s) {
+  EvolveArgs a;
+  int rc = make_args(h, n, ld, y, node_par, t_start, t_end, h_step, scale, status, counters, a);
+  if (rc != GAL_OK) return rc;
+  if (n_subset < 0 || n_subset > n || (subset == nullptr && n_subset != n)) return GAL_E_INVALID;
+  if (n_subset == 0) return GAL_OK;
+  a.n_claim = n_subset;
+  a.order = subset;
+  if (h->flags & GAL_FLAG_COST_ORDER) {
+    const int threads = 256;
+    hipLaunchKernelGGL(cost_probe_kernel, dim3((n_subset + threads - 1) / threads), dim3(threads), 0, s, n_subset,
+                       subset, ld, (const double*)y, node_par, t_start, t_end, (const double*)scale,
+                       (const double*)h->d_params, (const double*)h->d_table, h->D, h->cd, a.mode,
+                       (h->flags & GAL_FLAG_SCALES_PROVIDED) ? 1 : 0, h->d_key, h->d_index);
+    HIP_TRY(hipGetLastError());
+    size_t tb = h->sort_temp_bytes;
+    HIP_TRY(hipcub::DeviceRadixSort::SortPairsDescending(h->d_sort_temp, tb, h->d_key, h->d_key_sorted, h->d_index,
+                                                         h->d_order, n_subset, 0, 32, s));
+    a.order = h->d_order;
+  }
+  return launch_evolve(h, a, s);
+}

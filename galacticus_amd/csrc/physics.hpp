@@ -89,6 +89,22 @@ struct NodePar {
 
 __device__ __forceinline__ double sq(double x) { return x * x; }
 
+__device__ __forceinline__ NodePar load_node_par(const double* __restrict__ np, int ld, int i) {
+  NodePar p;
+  p.t_ref = np[GAL_NP_TIME_REFERENCE * (size_t)ld + i];
+  p.m_ref = np[GAL_NP_MASS_REFERENCE * (size_t)ld + i];
+  p.m_rate = np[GAL_NP_MASS_RATE * (size_t)ld + i];
+  p.spin_rate = np[GAL_NP_SPIN_RATE * (size_t)ld + i];
+  p.satellite = np[GAL_NP_IS_SATELLITE * (size_t)ld + i] != 0.0;
+  return p;
+}
+
+// isNonNegative attributes of the state variables (component _class.F90 XML).
+__device__ __forceinline__ bool var_non_negative(int v) {
+  return !(v == GAL_V_HOT_ANGULAR_MOMENTUM || v == GAL_V_HOT_OUTFLOWED_ANGULAR_MOMENTUM ||
+           v == GAL_V_HOT_ENERGY_RADIATED || v == GAL_V_BH_RADIAL_POSITION || v == GAL_V_SPIN_ANGULAR_MOMENTUM);
+}
+
 // x^(n2/2) for integer n2 given sqrt(x); wave-uniform n2.
 __device__ __forceinline__ double pow_half(double x, double sqrt_x, int n2) {
   const int m = n2 < 0 ? -n2 : n2;

@@ -166,3 +166,97 @@ def closed_box(timescale=0.25, outflow=0.0, recycled=0.4, yield_=0.025):
     npar[NP["TIME_REFERENCE"], 0] = 12.47
     npar[NP["MASS_REFERENCE"], 0] = 1.0e12
     return p, NodeBatch(y, npar, np.array([12.47]), np.array([13.47]), -np.ones(1))
+
+
+def hubble_time(t, om=0.3, ol=0.7, h0=70.0):
+    """1/H(t) in Gyr for flat LCDM."""
+    a = cosmic_time_to_scale_factor(t, om, ol, h0)
+    return MPC_PER_KMS_GYR / (h0 * np.sqrt(om / a ** 3 + ol))
+
+
+def simple_timestep(t, om=0.3, ol=0.7, h0=70.0, relative=0.1, absolute=1.0):
+    """mergerTreeEvolveTimestepSimple (merger_trees/evolve/timesteps/simple.F90:124-155):
+    min(timeStepRelative / H(t), timeStepAbsolute)."""
+    return np.minimum(relative * hubble_time(t, om, ol, h0), absolute)
+
+
+@dataclasses.dataclass
+class HaloHistories:
+    """Initial galaxies on growing halos, evolved interval by interval to z=0."""
+    batch: NodeBatch       # state at each history's start; t_end = time at z=0
+    t_final: float
+    seed: int
+
+    def intervals(self, t, rng):
+        """Next interval end for histories now at time t: the simple timestep,
+        shortened by a uniform factor in [0.25, 1] that stands in for the
+        clipping of intervals at the times of tree events, then clipped at z=0."""
+        p = gp.physics_params()
+        om, ol, h0 = p[gp.P["OmegaMatter"]], p[gp.P["OmegaDarkEnergy"]], p[gp.P["HubbleConstant"]]
+        dt = simple_timestep(t, om, ol, h0) * rng.uniform(0.25, 1.0, np.shape(t))
+        return np.minimum(t + dt, self.t_final)
+
+
+def halo_histories(n: int, seed: int = 0, params=None, start=None) -> HaloHistories:
+    """Node histories for the surrogate: each node's halo grows linearly from
+    M_s at its start time t_s to M_0 at z=0, with M_0 log-uniform in
+    [1e10, 1e13] Msun and log10(M_0/M_s) uniform in [0.3, 1.5] scaled by the
+    fraction of the z=5..0 time span still ahead; t_s is uniform between z=5
+    and 1 Gyr before z=0 (or fixed at ``start``).  The galaxy at t_s is drawn as
+    in ``isolated_halos`` with the halo at t_s."""
+    p = gp.physics_params() if params is None else params
+    om, ol, h0 = p[gp.P["OmegaMatter"]], p[gp.P["OmegaDarkEnergy"]], p[gp.P["HubbleConstant"]]
+    fb = p[gp.P["OmegaBaryon"]] / om
+    t_final = float(scale_factor_to_cosmic_time(1.0, om, ol, h0))
+    t_z5 = float(scale_factor_to_cosmic_time(1.0 / 6.0, om, ol, h0))
+    rng = np.random.Generator(np.random.PCG64(seed))
+    ts = rng.uniform(t_z5, t_final - 1.0, n) if start is None else np.full(n, float(start))
+    m0 = 10.0 ** rng.uniform(10.0, 13.0, n)
+    ahead = (t_final - ts) / (t_final - t_z5)
+    ms = m0 * 10.0 ** (-rng.uniform(0.3, 1.5, n) * ahead)
+    m_rate = (m0 - ms) / (t_final - ts)
+    f_hot = rng.uniform(0.05, 0.17, n)
+    metal = 10.0 ** rng.uniform(-4.0, math.log10(0.02), n)
+    spin = np.clip(np.exp(rng.normal(math.log(0.035), 0.5, n)), 0.01, 0.1)
+    f_out = rng.uniform(0.0, 0.02, n)
+    f_dg = rng.uniform(0.0, 0.1, n)
+    f_ds = rng.uniform(0.0, 0.1, n)
+    f_sg = rng.uniform(0.0, 0.02, n) * (rng.uniform(size=n) < 0.5)
+    f_ss = rng.uniform(0.0, 0.1, n) * (rng.uniform(size=n) < 0.5)
+    bh_spin = rng.uniform(0.0, 0.9, n)
+    r_vir, v_vir = virial(ms, ts, om, ol, h0)
+    j_spin = spin * ms * v_vir * r_vir
+    r_disk = p[gp.P["galacticStructureSolver.factor"]] * r_vir * spin
+    y = np.zeros((NV, n))
+    mb = fb * ms
+    y[V["HOT_MASS"]] = f_hot * ms
+    y[V["HOT_ABUNDANCES"]] = metal * y[V["HOT_MASS"]]
+    y[V["HOT_ANGULAR_MOMENTUM"]] = y[V["HOT_MASS"]] * j_spin / ms
+    y[V["HOT_OUTFLOWED_MASS"]] = f_out * mb
+    y[V["HOT_OUTFLOWED_ABUNDANCES"]] = metal * y[V["HOT_OUTFLOWED_MASS"]]
+    y[V["HOT_OUTFLOWED_ANGULAR_MOMENTUM"]] = y[V["HOT_OUTFLOWED_MASS"]] * j_spin / ms
+    y[V["HOT_OUTER_RADIUS"]] = r_vir
+    y[V["DISK_MASS_GAS"]] = f_dg * mb
+    y[V["DISK_MASS_STELLAR"]] = f_ds * mb
+    y[V["DISK_MASS_STELLAR_FORMED"]] = y[V["DISK_MASS_STELLAR"]]
+    y[V["DISK_FRACTION_MASS_RETAINED"]] = 1.0
+    y[V["DISK_ABUNDANCES_GAS"]] = metal * y[V["DISK_MASS_GAS"]]
+    y[V["DISK_ABUNDANCES_STELLAR"]] = metal * y[V["DISK_MASS_STELLAR"]]
+    y[V["DISK_ANGULAR_MOMENTUM"]] = (y[V["DISK_MASS_GAS"]] + y[V["DISK_MASS_STELLAR"]]) * 2.0 * r_disk * v_vir
+    y[V["SPHEROID_MASS_GAS"]] = f_sg * mb
+    y[V["SPHEROID_MASS_STELLAR"]] = f_ss * mb
+    y[V["SPHEROID_MASS_STELLAR_FORMED"]] = y[V["SPHEROID_MASS_STELLAR"]]
+    y[V["SPHEROID_ABUNDANCES_GAS"]] = metal * y[V["SPHEROID_MASS_GAS"]]
+    y[V["SPHEROID_ABUNDANCES_STELLAR"]] = metal * y[V["SPHEROID_MASS_STELLAR"]]
+    y[V["SPHEROID_ANGULAR_MOMENTUM"]] = (y[V["SPHEROID_MASS_GAS"]] + y[V["SPHEROID_MASS_STELLAR"]]) * 0.1 * r_disk * v_vir
+    y[V["BH_MASS"]] = 100.0 + 1.0e-3 * y[V["SPHEROID_MASS_STELLAR"]]
+    y[V["BH_SPIN"]] = bh_spin
+    y[V["SPIN_ANGULAR_MOMENTUM"]] = j_spin
+    y[V["SATELLITE_BOUND_MASS"]] = ms
+    npar = np.zeros((gp.NPARAM_NODE, n))
+    npar[NP["TIME_REFERENCE"]] = ts
+    npar[NP["MASS_REFERENCE"]] = ms
+    npar[NP["MASS_RATE"]] = m_rate
+    npar[NP["SPIN_RATE"]] = j_spin * m_rate / ms
+    npar[NP["IS_SATELLITE"]] = 0.0
+    return HaloHistories(NodeBatch(y, npar, ts, np.full(n, t_final), -np.ones(n)), t_final, seed)

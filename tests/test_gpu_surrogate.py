@@ -1,0 +1,278 @@
+"""GPU tests of the learned surrogate (include/gal_surrogate.h) through the C ABI.
+
+Bars:
+  * network forward and gradients against the CPU restatement with the same
+    bf16 rounding points (oracle/mlp.py): outputs within 2e-2 of their scale,
+    gradients within 3e-2 in norm per layer (the two differ by fp32 summation
+    order and the exp of SiLU, which can flip a bf16 rounding);
+  * the gradient against a finite difference of the GPU's own loss along the
+    gradient direction (15%: bf16 weights quantise the perturbation);
+  * Adam: the GPU update equals the CPU Adam applied to the GPU gradient
+    (rtol 1e-5);
+  * hipGraph replay and eager launches give bit-identical parameters;
+  * encoding: GPU inputs/targets equal the oracle's to fp32 rounding;
+  * routing: a node the gate rejects gets exactly the plain integrator's
+    result (bit-exact state, step size and status); an accepted node gets the
+    decoded network output (rtol 1e-10).
+"""
+import ctypes
+import math
+
+import numpy as np
+import pytest
+
+import oracle
+from oracle import mlp as om
+from galacticus_amd import params as gp
+from galacticus_amd import workloads as wl
+
+pytestmark = pytest.mark.gpu
+
+UNIT_ENERGY = 1.98847e30 * 1.0e6 / 1.0e-7 / (1.0e9 * 3.15581497635456e7)
+
+
+@pytest.fixture(scope="module")
+def torch_cuda():
+    import torch
+    assert torch.cuda.is_available(), "GPU tests need a GPU"
+    return torch
+
+
+def _bits(torch, x):
+    return torch.from_numpy(om.bf16_bits(x.astype(np.float32)).view(np.int16).copy()).cuda()
+
+
+def _loss_config():
+    from galacticus_amd.surrogate import TRAINED_VARS, HEAD_VARS
+    rng = np.random.default_rng(11)
+    w = np.zeros(30, dtype=np.float32)
+    w[TRAINED_VARS] = 1.0
+    sigma = rng.uniform(0.5, 2.0, 30).astype(np.float32)
+    hv = np.zeros(30, dtype=np.int32)
+    hv[HEAD_VARS] = 1
+    return w, sigma, hv
+
+
+def _net(batch=512, seed=0):
+    from galacticus_amd.surrogate import SurrogateMLP, init_params
+    rng = np.random.default_rng(seed + 100)
+    p = init_params(seed)
+    p = (p + rng.normal(0, 0.02, p.size)).astype(np.float32)
+    m = SurrogateMLP(batch, params=p)
+    w, sigma, hv = _loss_config()
+    m.set_loss(w, sigma, hv, 0.1)
+    return m, p
+
+
+def test_forward_matches_oracle(torch_cuda):
+    torch = torch_cuda
+    m, p = _net()
+    rng = np.random.default_rng(1)
+    x = om.bf16(rng.normal(0, 1, (300, 32)).astype(np.float32))
+    out = m.forward(_bits(torch, x)).cpu().numpy()
+    ref, _ = om.forward_bf16(p, x)
+    err = np.abs(out[:, :31] - ref)
+    print("forward max err", err.max(), "scale", np.abs(ref).max(), flush=True)
+    assert err.max() <= 2e-2 * np.abs(ref).max()
+    assert np.median(err) <= 1e-3 * np.abs(ref).max()
+
+
+def test_gradient_matches_oracle(torch_cuda):
+    torch = torch_cuda
+    m, p = _net()
+    rng = np.random.default_rng(2)
+    x = om.bf16(rng.normal(0, 1, (300, 32)).astype(np.float32))
+    t = rng.normal(0, 1, (300, 32)).astype(np.float32)
+    g, loss = m.gradient(_bits(torch, x), torch.from_numpy(t).cuda())
+    g = g.cpu().numpy()
+    w, sigma, hv = _loss_config()
+    out, cache = om.forward_bf16(p, x)
+    lref, g_out = om.loss_grad(out, t, w, sigma, hv.astype(bool), 0.1)
+    gref = om.backward_bf16(p, cache, g_out)
+    print("loss gpu", float(loss), "cpu", lref, flush=True)
+    assert abs(float(loss) - lref) <= 1e-2 * abs(lref)
+    o = 0
+    for a, b in zip(om.FAN_IN, om.FAN_OUT):
+        for size in (a * b, b):
+            d = np.linalg.norm(g[o:o + size] - gref[o:o + size]) / np.linalg.norm(gref[o:o + size])
+            print("layer block", o, size, "rel err", d, flush=True)
+            assert d <= 3e-2, (o, size, d)
+            o += size
+
+
+def test_gradient_finite_difference_on_gpu(torch_cuda):
+    torch = torch_cuda
+    m, p = _net()
+    rng = np.random.default_rng(3)
+    xb = _bits(torch, om.bf16(rng.normal(0, 1, (256, 32)).astype(np.float32)))
+    t = torch.from_numpy(rng.normal(0, 1, (256, 32)).astype(np.float32)).cuda()
+    g, l0 = m.gradient(xb, t)
+    g = g.cpu().numpy().astype(np.float64)
+    d = g / np.linalg.norm(g)
+    eps = 2e-2
+    losses = []
+    for s in (+1, -1):
+        m.set_params((p + s * eps * d).astype(np.float32))
+        _, l = m.gradient(xb, t)
+        losses.append(float(l))
+    fd = (losses[0] - losses[1]) / (2 * eps)
+    print("directional fd", fd, "analytic", np.linalg.norm(g), flush=True)
+    assert abs(fd - np.linalg.norm(g)) <= 0.15 * np.linalg.norm(g)
+
+
+def test_adam_step_matches_oracle(torch_cuda):
+    torch = torch_cuda
+    m, p = _net(batch=512)
+    rng = np.random.default_rng(4)
+    x = _bits(torch, om.bf16(rng.normal(0, 1, (512, 32)).astype(np.float32)))
+    t = torch.from_numpy(rng.normal(0, 1, (512, 32)).astype(np.float32)).cuda()
+    perm = torch.arange(512, dtype=torch.int32, device="cuda")
+    g, _ = m.gradient(x, t)
+    g = g.cpu().numpy()
+    m.set_adam(lr=1e-3, lr_min=1e-5, total_steps=100)
+    m.set_dataset(x, t, perm)
+    m.train_steps(1, use_graph=False)
+    torch.cuda.synchronize()
+    p1 = m.get_params()
+    z = np.zeros_like(p)
+    ref, _, _ = om.adam(p, z, z, g, 0, 1e-3, 1e-5, 0.9, 0.999, 1e-8, 100)
+    assert np.allclose(p1, ref, rtol=1e-5, atol=1e-7)
+    assert np.abs(p1 - p).max() > 5e-4
+
+
+def test_graph_replay_equals_eager(torch_cuda):
+    torch = torch_cuda
+    rng = np.random.default_rng(5)
+    x = _bits(torch, om.bf16(rng.normal(0, 1, (2048, 32)).astype(np.float32)))
+    t = torch.from_numpy(rng.normal(0, 1, (2048, 32)).astype(np.float32)).cuda()
+    perm = torch.from_numpy(rng.permutation(2048).astype(np.int32)).cuda()
+    res = []
+    for use_graph in (True, False):
+        m, _ = _net(batch=512, seed=9)
+        m.set_adam(lr=1e-3, total_steps=12)
+        m.set_dataset(x, t, perm)
+        tr = [m.train_steps(4, use_graph=use_graph) for _ in range(3)]
+        torch.cuda.synchronize()
+        res.append((m.get_params(), torch.cat(tr).cpu().numpy()))
+    assert np.array_equal(res[0][0], res[1][0])
+    assert np.array_equal(res[0][1], res[1][1])
+
+
+def test_training_fits_a_smooth_map(torch_cuda):
+    torch = torch_cuda
+    from galacticus_amd.surrogate import SurrogateMLP
+    rng = np.random.default_rng(6)
+    n = 16384
+    xs = rng.normal(0, 1, (n, 32)).astype(np.float32)
+    A = rng.normal(0, 0.3, (32, 32)).astype(np.float32)
+    ts = np.zeros((n, 32), dtype=np.float32)
+    ts[:, :30] = np.tanh(xs @ A)[:, :30]
+    m = SurrogateMLP(1024, seed=1)
+    w = np.ones(30, dtype=np.float32)
+    m.set_loss(w, np.ones(30, dtype=np.float32), np.ones(30, dtype=np.int32), 0.01)
+    m.set_adam(lr=2e-3, lr_min=1e-4, total_steps=16 * 20)
+    x = _bits(torch, xs)
+    t = torch.from_numpy(ts).cuda()
+    losses = []
+    for ep in range(20):
+        perm = torch.from_numpy(rng.permutation(n).astype(np.int32)).cuda()
+        m.set_dataset(x, t, perm)
+        losses.append(m.train_steps(16).cpu().numpy())
+    losses = np.concatenate(losses)
+    print("loss first", losses[:4].mean(), "last", losses[-4:].mean(), flush=True)
+    assert losses[-4:].mean() < 0.1 * losses[:4].mean()
+
+
+def _halo(params, table, node_par_col, t, y_col):
+    hp = oracle.halo_properties(params, table, node_par_col, t, y_col)
+    mass = node_par_col[1] + node_par_col[2] * (t - node_par_col[0])
+    return dict(mass=mass, r_vir=hp["r_vir"], v_vir=hp["v_vir"], t_dyn=hp["t_dyn"], a=hp["a"])
+
+
+def test_encode_matches_oracle(torch_cuda, cooling_table):
+    torch = torch_cuda
+    from galacticus_amd.evolver import NodeEvolverBatched
+    from galacticus_amd.surrogate import PairSet, encode
+    p = gp.physics_params()
+    hist = wl.halo_histories(96, seed=3, params=p)
+    b = hist.batch
+    ev = NodeEvolverBatched(b.n, p, cooling_table)
+    d = lambda a: torch.from_numpy(np.ascontiguousarray(a)).cuda()
+    t1 = np.minimum(b.t_start + 0.5, hist.t_final)
+    y1 = d(b.y)
+    ev.evolve_device(y1, d(b.node_par), d(b.t_start), d(t1), d(b.h_step))
+    pairs = PairSet(d(b.y), y1, d(b.node_par), d(b.t_start), d(t1))
+    _, x_raw, t_raw = encode(ev, pairs, raw=True)
+    x_raw, t_raw, y1h = x_raw.cpu().numpy(), t_raw.cpu().numpy(), y1.cpu().numpy()
+    fb = p[gp.P["OmegaBaryon"]] / p[gp.P["OmegaMatter"]]
+    a_reion = 1.0 / (1.0 + p[gp.P["redshiftReionization"]])
+    for i in range(b.n):
+        halo = _halo(p, cooling_table, b.node_par[:, i], b.t_start[i], b.y[:, i])
+        xr, tr = om.encode_raw(b.y[:, i], y1h[:, i], b.t_start[i], t1[i], halo, b.node_par[:, i], fb, UNIT_ENERGY,
+                               p[gp.P["velocitySuppressionReionization"]], a_reion ** 3)
+        assert np.allclose(x_raw[i], xr, rtol=2e-6, atol=2e-6), (i, np.abs(x_raw[i] - xr).max())
+        assert np.allclose(t_raw[i, :30], tr, rtol=2e-5, atol=2e-6), (i, np.abs(t_raw[i, :30] - tr).max())
+
+
+def _surrogate_setup(torch, cooling_table, n=2000, seed=5):
+    from galacticus_amd.evolver import NodeEvolverBatched
+    from galacticus_amd.surrogate import SurrogateMLP, PairSet, encode, fit_normalization
+    p = gp.physics_params()
+    hist = wl.halo_histories(n, seed=seed, params=p)
+    b = hist.batch
+    ev = NodeEvolverBatched(n, p, cooling_table)
+    t1 = np.minimum(b.t_start + np.random.default_rng(seed).uniform(0.1, 1.0, n), hist.t_final)
+    d = lambda a: torch.from_numpy(np.ascontiguousarray(a)).cuda()
+    pairs = PairSet(d(b.y), d(b.y), d(b.node_par), d(b.t_start), d(t1))
+    _, x_raw, t_raw = encode(ev, pairs, raw=True)
+    norm = fit_normalization(x_raw, t_raw)
+    mlp = SurrogateMLP(1024, seed=2)
+    return p, b, t1, ev, mlp, norm, d
+
+
+def test_rejected_nodes_get_the_integrator_result(torch_cuda, cooling_table):
+    torch = torch_cuda
+    from galacticus_amd.surrogate import SurrogateEvolver, GateConfig
+    p, b, t1, ev, mlp, norm, d = _surrogate_setup(torch, cooling_table)
+    y_ref, h_ref = d(b.y), d(b.h_step)
+    st_ref = ev.evolve_device(y_ref, d(b.node_par), d(b.t_start), d(t1), h_ref)
+    se = SurrogateEvolver(ev, mlp, norm, GateConfig(error_max=1e-30))
+    y, h = d(b.y), d(b.h_step)
+    st, acc, n_rej = se.evolve_device(y, d(b.node_par), d(b.t_start), d(t1), h)
+    torch.cuda.synchronize()
+    assert n_rej == b.n and int(acc.sum()) == 0
+    assert torch.equal(y, y_ref) and torch.equal(h, h_ref) and torch.equal(st, st_ref)
+
+
+def test_accepted_nodes_get_the_decoded_network_output(torch_cuda, cooling_table):
+    torch = torch_cuda
+    from galacticus_amd.surrogate import SurrogateEvolver, GateConfig, PairSet, encode
+    p, b, t1, ev, mlp, norm, d = _surrogate_setup(torch, cooling_table, n=600, seed=8)
+    pairs = PairSet(d(b.y), None, d(b.node_par), d(b.t_start), d(t1))
+    x, _, _ = encode(ev, pairs, norm=norm, with_target=False)
+    out = mlp.forward(x).cpu().numpy().astype(np.float64)
+    nm = norm.cpu().numpy().astype(np.float64)
+    y_rk = d(b.y)
+    ev.evolve_device(y_rk, d(b.node_par), d(b.t_start), d(t1), d(b.h_step))
+    se = SurrogateEvolver(ev, mlp, norm, GateConfig(error_max=1e300, mass_tolerance=1e300, metal_tolerance=1e300))
+    y, h = d(b.y), d(b.h_step)
+    st, acc, n_rej = se.evolve_device(y, d(b.node_par), d(b.t_start), d(t1), h)
+    yh, acc, y_rk = y.cpu().numpy(), acc.cpu().numpy(), y_rk.cpu().numpy()
+    assert (st.cpu().numpy() == 0).all()
+    fb = p[gp.P["OmegaBaryon"]] / p[gp.P["OmegaMatter"]]
+    nonneg = [v for v in range(30) if v not in (2, 4, 11, 27, 28)]
+    n_acc = 0
+    for i in range(b.n):
+        halo = _halo(p, cooling_table, b.node_par[:, i], b.t_start[i], b.y[:, i])
+        dd = nm[2, :30] + nm[3, :30] * out[i, :30]
+        y1 = om.decode(b.y[:, i], dd, t1[i] - b.t_start[i], halo, b.node_par[:, i], fb, UNIT_ENERGY)
+        ok = bool(np.isfinite(y1).all() and (y1[nonneg] >= 0).all())
+        if acc[i]:
+            n_acc += 1
+            assert ok, i
+            assert np.allclose(yh[:, i], y1, rtol=1e-10, atol=0), (i, np.abs(yh[:, i] - y1).max())
+        else:
+            assert not ok, i
+            assert np.array_equal(yh[:, i], y_rk[:, i]), i
+    print("accepted", n_acc, "of", b.n, flush=True)
+    assert 0 < n_acc < b.n

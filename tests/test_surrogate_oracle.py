@@ -1,0 +1,155 @@
+"""CPU tests of the surrogate's oracle (oracle/mlp.py) and host logic.
+
+The network's CPU restatement is pinned by what the mathematics fixes:
+  * the float64 backward pass against central finite differences;
+  * the bf16-rounded backward pass (the GPU's rounding points) against the
+    float64 one, at bf16 resolution;
+  * bf16 rounding against known bit patterns (nearest, ties to even);
+  * Adam against its closed form for the first step (|step| = lr);
+  * the target encoding against its inverse (decode(encode) = identity).
+"""
+import math
+
+import numpy as np
+import pytest
+
+from oracle import mlp as om
+
+
+def _setup(n=6, seed=0):
+    from galacticus_amd.surrogate import init_params, TRAINED_VARS, HEAD_VARS
+    rng = np.random.default_rng(seed)
+    p = init_params(seed).astype(np.float64)
+    p += rng.normal(0, 0.02, p.size)           # non-zero biases
+    x = rng.normal(0, 1, (n, 32))
+    t = rng.normal(0, 1, (n, 32))
+    w = np.zeros(30)
+    w[TRAINED_VARS] = 1.0
+    sigma = rng.uniform(0.5, 2.0, 30)
+    hm = np.zeros(30, dtype=bool)
+    hm[HEAD_VARS] = True
+    return p, x, t, w, sigma, hm
+
+
+def test_backward64_matches_finite_differences():
+    p, x, t, w, sigma, hm = _setup()
+    out, cache = om.forward64(p, x)
+    g, e = om.backward64(p, cache, out, t, w, sigma, hm, 0.1)
+    rng = np.random.default_rng(3)
+    idx = np.concatenate([rng.choice(p.size, 60, replace=False),
+                          # the last layer's weights and biases, and the head column
+                          [p.size - 1, p.size - 31 + 30, p.size - 31 - 1]])
+    for i in idx:
+        h = 1e-6 * max(1.0, abs(p[i]))
+        pp, pm = p.copy(), p.copy()
+        pp[i] += h
+        pm[i] -= h
+        lp, _ = om.loss64(om.forward64(pp, x)[0], t, w, sigma, hm, 0.1, e_fixed=e)
+        lm, _ = om.loss64(om.forward64(pm, x)[0], t, w, sigma, hm, 0.1, e_fixed=e)
+        fd = (lp - lm) / (2 * h)
+        assert abs(fd - g[i]) <= 1e-6 * max(1.0, abs(g[i])) + 1e-9, (i, fd, g[i])
+
+
+def test_backward_bf16_tracks_backward64():
+    p, x, t, w, sigma, hm = _setup(n=64)
+    x32 = om.bf16(x.astype(np.float32))
+    p32 = p.astype(np.float32)
+    out, cache = om.forward_bf16(p32, x32)
+    loss, g_out = om.loss_grad(out, t.astype(np.float32), w, sigma, hm, 0.1)
+    g16 = om.backward_bf16(p32, cache, g_out)
+    out64, c64 = om.forward64(om.bf16(p32).astype(np.float64), x32.astype(np.float64))
+    g64, _ = om.backward64(om.bf16(p32).astype(np.float64), c64, out64, t, w, sigma, hm, 0.1)
+    # biases of the bf16 path are unrounded fp32; only the rounding of weights,
+    # activations and gradients separates the two.
+    o = 0
+    for a, b in zip(om.FAN_IN, om.FAN_OUT):
+        for size in (a * b, b):
+            s16, s64 = g16[o:o + size], g64[o:o + size]
+            assert np.linalg.norm(s16 - s64) <= 3e-2 * np.linalg.norm(s64), (o, size)
+            o += size
+    assert np.allclose(out, out64, rtol=3e-2, atol=3e-2)
+
+
+def test_bf16_rounding_known_bits():
+    vals = np.array([1.0, math.pi, 1.0 + 2 ** -8, 1.0 + 3 * 2 ** -8, -2.5, 65504.0, 1e-40], dtype=np.float32)
+    bits = om.bf16_bits(vals)
+    assert [int(b) for b in bits[:6]] == [0x3F80, 0x4049, 0x3F80, 0x3F82, 0xC020, 0x4780]
+    assert om.from_bf16_bits(bits)[0] == 1.0
+    assert np.isnan(om.bf16(np.array([np.nan], dtype=np.float32)))[0]
+
+
+def test_adam_first_step_closed_form():
+    rng = np.random.default_rng(0)
+    p = rng.normal(0, 1, 100).astype(np.float32)
+    g = rng.normal(0, 1, 100).astype(np.float32)
+    z = np.zeros(100, dtype=np.float32)
+    p1, m1, v1 = om.adam(p, z, z, g, 0, 1e-3, 1e-5, 0.9, 0.999, 1e-8, 1000)
+    lr = 1e-5 + 0.5 * (1e-3 - 1e-5) * (1 + math.cos(math.pi / 1000))
+    # bias-corrected moments are g and g^2: the first step is lr * sign(g)
+    # (float32 state: p - p1 carries the rounding of p ~ 1 at a step of 1e-3)
+    assert np.allclose(p.astype(np.float64) - p1, lr * np.sign(g) * np.abs(g) / (np.abs(g) + 1e-8), rtol=2e-4)
+    assert np.allclose(m1, 0.1 * g, rtol=1e-5) and np.allclose(v1, 0.001 * g * g, rtol=1e-4)
+
+
+def test_cosine_schedule_end_points():
+    z = np.zeros(1, dtype=np.float32)
+    g = np.ones(1, dtype=np.float32)
+    # at the last step the learning rate is lr_min
+    p1, _, _ = om.adam(np.zeros(1, np.float32), z, z, g, 999, 1e-3, 1e-5, 0.9, 0.999, 1e-8, 1000)
+    # at t = T, bias corrections: m_hat = 0.1/(1-0.9^1000) ~ 0.1, v_hat ~ 0.001/(1-0.999^1000)
+    mh = 0.1 / (1 - 0.9 ** 1000)
+    vh = 0.001 / (1 - 0.999 ** 1000)
+    assert abs(-p1[0] - 1e-5 * mh / (math.sqrt(vh) + 1e-8)) < 1e-9
+
+
+def test_encoding_roles_match_state_layout():
+    from galacticus_amd import workloads as wl
+    from galacticus_amd.surrogate import INPUT_VARS, TRAINED_VARS
+    V = wl.V
+    assert om.INPUT_VARS == INPUT_VARS
+    assert om.VAR_ROLES[V["HOT_ENERGY_RADIATED"]] == om.ROLE_ENERGY
+    assert om.VAR_ROLES[V["SPIN_ANGULAR_MOMENTUM"]] == om.ROLE_SPIN
+    for k in ("HOT_OUTER_RADIUS", "HOT_STRIPPED_MASS", "HOT_STRIPPED_ABUNDANCES", "DISK_FRACTION_MASS_RETAINED",
+              "BH_RADIAL_POSITION", "SATELLITE_BOUND_MASS"):
+        assert om.VAR_ROLES[V[k]] == om.ROLE_CONST, k
+    for k in ("DISK_MASS_STELLAR", "SPHEROID_MASS_GAS", "HOT_UNACCRETED_MASS"):
+        assert om.VAR_ROLES[V[k]] == om.ROLE_MASS
+    for k in ("DISK_ABUNDANCES_GAS", "HOT_OUTFLOWED_ABUNDANCES"):
+        assert om.VAR_ROLES[V[k]] == om.ROLE_METAL
+    assert sorted(set(TRAINED_VARS)) == TRAINED_VARS and len(TRAINED_VARS) == 23
+    assert len(INPUT_VARS) == 22
+
+
+def test_encode_decode_round_trip():
+    from galacticus_amd import params as gp, workloads as wl
+    p = gp.physics_params()
+    b = wl.isolated_halos(20, seed=4, params=p)
+    rng = np.random.default_rng(5)
+    fb = p[gp.P["OmegaBaryon"]] / p[gp.P["OmegaMatter"]]
+    for i in range(b.n):
+        y0 = b.y[:, i]
+        y1 = y0 * rng.uniform(0.5, 2.0, 30)
+        y1[om.VAR_ROLES.index(om.ROLE_SPIN)] = y0[28] + b.node_par[3, i] * (b.t_end[i] - b.t_start[i])
+        for v, r in enumerate(om.VAR_ROLES):
+            if r == om.ROLE_CONST:
+                y1[v] = y0[v]
+        r_vir, v_vir = wl.virial(b.node_par[1, i], b.t_start[i])
+        halo = dict(mass=b.node_par[1, i], r_vir=float(r_vir), v_vir=float(v_vir), t_dyn=1.0, a=0.5)
+        _, t = om.encode_raw(y0, y1, b.t_start[i], b.t_end[i], halo, b.node_par[:, i], fb, 1.0, 30.0, 1e-3)
+        back = om.decode(y0, t, b.t_end[i] - b.t_start[i], halo, b.node_par[:, i], fb, 1.0)
+        assert np.allclose(back, y1, rtol=1e-12, atol=0), i
+
+
+def test_gate_struct_and_normalization():
+    torch = pytest.importorskip("torch")
+    from galacticus_amd.surrogate import GateConfig, fit_normalization
+    g = GateConfig(error_max=0.02).c_struct()
+    assert abs(g.log_error_max - math.log(0.02)) < 1e-15
+    x = torch.randn(1000, 32)
+    x[:, 31] = 0.0
+    t = torch.randn(1000, 32) * 3.0
+    norm = fit_normalization(x, t)
+    assert norm.shape == (4, 32)
+    assert float(norm[1, 31]) == 1.0              # constant input: unit std
+    assert float(norm[2, 8]) == 0.0 and float(norm[3, 8]) == 1.0   # untrained target column
+    assert abs(float(norm[3, 0]) - 3.0) < 0.3
