@@ -388,10 +388,34 @@ __device__ __forceinline__ double var_scale(const NodeFrame& f, int role) {
   }
 }
 
+// The first-order (explicit Euler) increment of each trained variable in
+// asinh space, from one RHS evaluation at t0.  The network learns only what
+// this misses.
+__device__ __forceinline__ void euler_baseline(const double* __restrict__ P, const DerivedConstants& D,
+                                               const double* __restrict__ tab, const CoolingDesc& cd, int mode,
+                                               const NodePar& np, const NodeFrame& f, double t0, double dt,
+                                               const double (&y0)[kNV], double (&du)[kNV]) {
+  double rate[kNV];
+  rhs(P, D, tab, cd, mode, np, t0, y0, rate);
+#pragma unroll
+  for (int v = 0; v < kNV; ++v) {
+    const int role = var_role(v);
+    double d = 0.0;
+    if (role == R_ENERGY) {
+      d = asinh(rate[v] * dt / f.s_energy);
+    } else if (role != R_CONST && role != R_SPIN) {
+      const double s = var_scale(f, role);
+      d = asinh((y0[v] + rate[v] * dt) / s) - asinh(y0[v] / s);
+    }
+    du[v] = d;
+  }
+}
+
 __global__ __launch_bounds__(256) void encode_kernel(int n, int ld, const double* __restrict__ y0,
                                                      const double* __restrict__ y1, const double* __restrict__ npa,
                                                      const double* __restrict__ t0a, const double* __restrict__ t1a,
                                                      const double* __restrict__ P, DerivedConstants D,
+                                                     const double* __restrict__ tab, CoolingDesc cd, int mode,
                                                      const float* __restrict__ norm, uint16_t* __restrict__ x,
                                                      float* __restrict__ x_raw, float* __restrict__ target) {
   const int i = blockIdx.x * blockDim.x + threadIdx.x;
@@ -429,16 +453,20 @@ __global__ __launch_bounds__(256) void encode_kernel(int n, int ld, const double
     }
   }
   if (target && y1) {
+    double ys[kNV], du[kNV];
+#pragma unroll
+    for (int v = 0; v < kNV; ++v) ys[v] = y0[v * (size_t)ld + i];
+    euler_baseline(P, D, tab, cd, mode, np, f, t0, dt, ys, du);
     float* tg = target + (size_t)i * kOutLd;
     for (int v = 0; v < GAL_NVAR; ++v) {
       const int role = var_role(v);
-      const double a = y0[v * (size_t)ld + i], b = y1[v * (size_t)ld + i];
+      const double a = ys[v], b = y1[v * (size_t)ld + i];
       double d = 0.0;
       if (role == R_ENERGY) {
-        d = asinh((b - a) / f.s_energy);
+        d = asinh((b - a) / f.s_energy) - du[v];
       } else if (role != R_CONST && role != R_SPIN) {
         const double s = var_scale(f, role);
-        d = asinh(b / s) - asinh(a / s);
+        d = asinh(b / s) - asinh(a / s) - du[v];
       }
       float z = (float)d;
       if (norm) z = (z - norm[2 * kIn + v]) / norm[3 * kIn + v];
@@ -464,7 +492,10 @@ struct GateArgs {
   int* reject_count;
   int index_offset;   // global index of node 0 of this call (chunking)
   const double* P;
+  const double* tab;
   DerivedConstants D;
+  CoolingDesc cd;
+  int mode;
   gal_surrogate_gate gate;
 };
 
@@ -486,10 +517,12 @@ __global__ __launch_bounds__(256) void apply_kernel(GateArgs a) {
       const float* o = a.out + (size_t)i * kOutLd;
       const float* nm = a.norm;
       bool ok = (double)o[GAL_MLP_HEAD] <= a.gate.log_error_max;
+      double du[kNV];
+      euler_baseline(a.P, a.D, a.tab, a.cd, a.mode, np, f, t0, dt, y0, du);
 #pragma unroll
       for (int v = 0; v < kNV; ++v) {
         const int role = var_role(v);
-        const double d = (double)nm[2 * kIn + v] + (double)nm[3 * kIn + v] * (double)o[v];
+        const double d = du[v] + (double)nm[2 * kIn + v] + (double)nm[3 * kIn + v] * (double)o[v];
         if (role == R_CONST) {
           y1[v] = y0[v];
         } else if (role == R_SPIN) {
@@ -993,8 +1026,10 @@ int gal_surrogate_encode_d(gal_ode_batch* h, int n, int ld, const double* y0, co
   if (!h || n < 0 || ld < n || !h->have_physics) return GAL_E_INVALID;
   if (n == 0) return GAL_OK;
   if (!y0 || !node_par || !t0 || !t1 || (target && !y1)) return GAL_E_INVALID;
+  if (target && !h->have_table) return GAL_E_INVALID;
   hipLaunchKernelGGL(encode_kernel, dim3((n + 255) / 256), dim3(256), 0, (hipStream_t)stream, n, ld, y0, y1, node_par,
-                     t0, t1, (const double*)h->d_params, h->D, norm, x, x_raw, target);
+                     t0, t1, (const double*)h->d_params, h->D, (const double*)h->d_table, h->cd,
+                     (int)h->params[GAL_P_PHYSICS_MODE], norm, x, x_raw, target);
   HIP_TRY(hipGetLastError());
   return GAL_OK;
 }
@@ -1007,7 +1042,8 @@ static int apply_chunk(gal_ode_batch* h, const gal_surrogate_gate* gate, int n, 
   a.n = n; a.ld = ld; a.y = y; a.np = node_par; a.t0 = t0; a.t1 = t1; a.out = mlp_out; a.norm = norm;
   a.scale = scale; a.status = status; a.accepted = accepted; a.reject_list = reject_list;
   a.reject_count = reject_count; a.index_offset = index_offset;
-  a.P = h->d_params; a.D = h->D; a.gate = *gate;
+  a.P = h->d_params; a.tab = h->d_table; a.D = h->D; a.cd = h->cd; a.mode = (int)h->params[GAL_P_PHYSICS_MODE];
+  a.gate = *gate;
   hipLaunchKernelGGL(apply_kernel, dim3((n + 255) / 256), dim3(256), 0, s, a);
   HIP_TRY(hipGetLastError());
   return GAL_OK;
@@ -1017,7 +1053,7 @@ int gal_surrogate_apply_d(gal_ode_batch* h, const gal_surrogate_gate* gate, int 
                           const double* node_par, const double* t0, const double* t1, const float* mlp_out,
                           const float* norm, double* scale, int* status, int* accepted, int* reject_list,
                           int* reject_count, void* stream) {
-  if (!h || !gate || n < 0 || ld < n || !h->have_physics) return GAL_E_INVALID;
+  if (!h || !gate || n < 0 || ld < n || !h->have_physics || !h->have_table) return GAL_E_INVALID;
   if (n == 0) return GAL_OK;
   if (!y || !node_par || !t0 || !t1 || !mlp_out || !norm || !status || !accepted || !reject_list || !reject_count)
     return GAL_E_INVALID;

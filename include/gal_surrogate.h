@@ -26,7 +26,9 @@
  *   25 Mdot_halo dt / M_halo; 26 ln spin parameter; 27 r_outer / r_vir;
  *   28 V_vir/200; 29 reionization suppression flag; 30 ln(dt / t_dyn); 31 0.
  *   Targets 0..29: u_v(t1) - u_v(t0) with the t0 scales (the radiated
- *   energy uses asinh(dE / s_E)); column 30 is the error head.
+ *   energy uses asinh(dE / s_E)), minus the same quantity for the explicit
+ *   Euler step y0 + f(t0, y0) dt (one RHS evaluation): the network learns
+ *   the residual of a first-order step.  Column 30 is the error head.
  *   Variables with zero rate are carried over, the spin angular momentum is
  *   advanced analytically (its rate is the node's constant spin rate).
  * Normalisation array norm (device fp32, 4 x 32):

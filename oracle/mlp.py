@@ -206,9 +206,22 @@ def node_frame(mass, r_vir, v_vir, f_baryon, unit_energy):
             ROLE_ENERGY: unit_energy * mass * v_vir * v_vir * 1e-3, ROLE_BH: 1.0, ROLE_UNIT: 1.0}
 
 
-def encode_raw(y0, y1, t0, t1, halo, node_par, f_baryon, unit_energy, v_suppress, a3_reion):
+def _baseline(y0, rate, dt, sc):
+    """Asinh-space increment of the explicit Euler step y0 + rate*dt."""
+    du = np.zeros(30)
+    for v in range(30):
+        role = VAR_ROLES[v]
+        if role == ROLE_ENERGY:
+            du[v] = math.asinh(rate[v] * dt / sc[role])
+        elif role not in (ROLE_CONST, ROLE_SPIN):
+            du[v] = math.asinh((y0[v] + rate[v] * dt) / sc[role]) - math.asinh(y0[v] / sc[role])
+    return du
+
+
+def encode_raw(y0, y1, t0, t1, halo, node_par, f_baryon, unit_energy, v_suppress, a3_reion, rate=None):
     """Raw inputs (32) and raw targets (30) of one node.  halo: dict with
-    mass, r_vir, v_vir, t_dyn, a (expansion factor)."""
+    mass, r_vir, v_vir, t_dyn, a (expansion factor); rate: dy/dt at (t0, y0)
+    for the Euler baseline (zero if None)."""
     M, rv, vv = halo["mass"], halo["r_vir"], halo["v_vir"]
     sc = node_frame(M, rv, vv, f_baryon, unit_energy)
     dt = t1 - t0
@@ -229,18 +242,20 @@ def encode_raw(y0, y1, t0, t1, halo, node_par, f_baryon, unit_energy, v_suppress
     x[30] = math.log(dtp / halo["t_dyn"])
     t = np.zeros(30)
     if y1 is not None:
+        du = _baseline(y0, np.zeros(30) if rate is None else rate, dt, sc)
         for v in range(30):
             role = VAR_ROLES[v]
             if role == ROLE_ENERGY:
-                t[v] = math.asinh((y1[v] - y0[v]) / sc[role])
+                t[v] = math.asinh((y1[v] - y0[v]) / sc[role]) - du[v]
             elif role not in (ROLE_CONST, ROLE_SPIN):
-                t[v] = math.asinh(y1[v] / sc[role]) - math.asinh(y0[v] / sc[role])
+                t[v] = math.asinh(y1[v] / sc[role]) - math.asinh(y0[v] / sc[role]) - du[v]
     return x, t
 
 
-def decode(y0, d, dt, halo, node_par, f_baryon, unit_energy):
-    """Inverse of the target map: state at t1 from y0 and raw increments d."""
+def decode(y0, d, dt, halo, node_par, f_baryon, unit_energy, rate=None):
+    """Inverse of the target map: state at t1 from y0 and raw residuals d."""
     sc = node_frame(halo["mass"], halo["r_vir"], halo["v_vir"], f_baryon, unit_energy)
+    du = _baseline(y0, np.zeros(30) if rate is None else rate, dt, sc)
     y1 = np.array(y0, dtype=np.float64)
     for v in range(30):
         role = VAR_ROLES[v]
@@ -249,7 +264,7 @@ def decode(y0, d, dt, halo, node_par, f_baryon, unit_energy):
         if role == ROLE_SPIN:
             y1[v] = y0[v] + (0.0 if node_par[4] != 0.0 else node_par[3]) * dt
         elif role == ROLE_ENERGY:
-            y1[v] = y0[v] + sc[role] * math.sinh(d[v])
+            y1[v] = y0[v] + sc[role] * math.sinh(du[v] + d[v])
         else:
-            y1[v] = sc[role] * math.sinh(math.asinh(y0[v] / sc[role]) + d[v])
+            y1[v] = sc[role] * math.sinh(math.asinh(y0[v] / sc[role]) + du[v] + d[v])
     return y1

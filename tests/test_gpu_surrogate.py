@@ -13,7 +13,8 @@ Bars:
   * encoding: GPU inputs/targets equal the oracle's to fp32 rounding;
   * routing: a node the gate rejects gets exactly the plain integrator's
     result (bit-exact state, step size and status); an accepted node gets the
-    decoded network output (rtol 1e-10).
+    decoded network output (rtol 1e-7: the Euler baseline of the decoding
+    comes from the GPU RHS on one side and the oracle RHS on the other).
 """
 import ctypes
 import math
@@ -170,11 +171,11 @@ def test_training_fits_a_smooth_map(torch_cuda):
     m = SurrogateMLP(1024, seed=1)
     w = np.ones(30, dtype=np.float32)
     m.set_loss(w, np.ones(30, dtype=np.float32), np.ones(30, dtype=np.int32), 0.01)
-    m.set_adam(lr=2e-3, lr_min=1e-4, total_steps=16 * 20)
+    m.set_adam(lr=2e-3, lr_min=1e-4, total_steps=16 * 40)
     x = _bits(torch, xs)
     t = torch.from_numpy(ts).cuda()
     losses = []
-    for ep in range(20):
+    for ep in range(40):
         perm = torch.from_numpy(rng.permutation(n).astype(np.int32)).cuda()
         m.set_dataset(x, t, perm)
         losses.append(m.train_steps(16).cpu().numpy())
@@ -206,10 +207,11 @@ def test_encode_matches_oracle(torch_cuda, cooling_table):
     x_raw, t_raw, y1h = x_raw.cpu().numpy(), t_raw.cpu().numpy(), y1.cpu().numpy()
     fb = p[gp.P["OmegaBaryon"]] / p[gp.P["OmegaMatter"]]
     a_reion = 1.0 / (1.0 + p[gp.P["redshiftReionization"]])
+    rate = oracle.rhs(b.t_start, b.y, b.node_par, p, cooling_table)
     for i in range(b.n):
         halo = _halo(p, cooling_table, b.node_par[:, i], b.t_start[i], b.y[:, i])
         xr, tr = om.encode_raw(b.y[:, i], y1h[:, i], b.t_start[i], t1[i], halo, b.node_par[:, i], fb, UNIT_ENERGY,
-                               p[gp.P["velocitySuppressionReionization"]], a_reion ** 3)
+                               p[gp.P["velocitySuppressionReionization"]], a_reion ** 3, rate[:, i])
         assert np.allclose(x_raw[i], xr, rtol=2e-6, atol=2e-6), (i, np.abs(x_raw[i] - xr).max())
         assert np.allclose(t_raw[i, :30], tr, rtol=2e-5, atol=2e-6), (i, np.abs(t_raw[i, :30] - tr).max())
 
@@ -262,15 +264,17 @@ def test_accepted_nodes_get_the_decoded_network_output(torch_cuda, cooling_table
     fb = p[gp.P["OmegaBaryon"]] / p[gp.P["OmegaMatter"]]
     nonneg = [v for v in range(30) if v not in (2, 4, 11, 27, 28)]
     n_acc = 0
+    rate = oracle.rhs(b.t_start, b.y, b.node_par, p, cooling_table)
     for i in range(b.n):
         halo = _halo(p, cooling_table, b.node_par[:, i], b.t_start[i], b.y[:, i])
         dd = nm[2, :30] + nm[3, :30] * out[i, :30]
-        y1 = om.decode(b.y[:, i], dd, t1[i] - b.t_start[i], halo, b.node_par[:, i], fb, UNIT_ENERGY)
+        y1 = om.decode(b.y[:, i], dd, t1[i] - b.t_start[i], halo, b.node_par[:, i], fb, UNIT_ENERGY, rate[:, i])
         ok = bool(np.isfinite(y1).all() and (y1[nonneg] >= 0).all())
         if acc[i]:
             n_acc += 1
             assert ok, i
-            assert np.allclose(yh[:, i], y1, rtol=1e-10, atol=0), (i, np.abs(yh[:, i] - y1).max())
+            # the Euler baseline uses the GPU and CPU RHS, which agree to ~1e-9
+            assert np.allclose(yh[:, i], y1, rtol=1e-7, atol=0), (i, np.abs(yh[:, i] - y1).max())
         else:
             assert not ok, i
             assert np.array_equal(yh[:, i], y_rk[:, i]), i

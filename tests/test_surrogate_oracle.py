@@ -135,8 +135,9 @@ def test_encode_decode_round_trip():
                 y1[v] = y0[v]
         r_vir, v_vir = wl.virial(b.node_par[1, i], b.t_start[i])
         halo = dict(mass=b.node_par[1, i], r_vir=float(r_vir), v_vir=float(v_vir), t_dyn=1.0, a=0.5)
-        _, t = om.encode_raw(y0, y1, b.t_start[i], b.t_end[i], halo, b.node_par[:, i], fb, 1.0, 30.0, 1e-3)
-        back = om.decode(y0, t, b.t_end[i] - b.t_start[i], halo, b.node_par[:, i], fb, 1.0)
+        rate = rng.normal(0, 1, 30) * np.abs(y0) * 0.1
+        _, t = om.encode_raw(y0, y1, b.t_start[i], b.t_end[i], halo, b.node_par[:, i], fb, 1.0, 30.0, 1e-3, rate)
+        back = om.decode(y0, t, b.t_end[i] - b.t_start[i], halo, b.node_par[:, i], fb, 1.0, rate)
         assert np.allclose(back, y1, rtol=1e-12, atol=0), i
 
 
