@@ -14,7 +14,7 @@ ROOT = os.path.dirname(HERE)
 CSRC = os.path.join(HERE, "csrc")
 LIBDIR = os.path.join(HERE, "lib")
 LIB = os.path.join(LIBDIR, "libgalacticus_amd.so")
-SOURCES = ["gal_ode.hip", "surrogate.hip"]
+SOURCES = ["gal_ode.hip", "surrogate.hip", "tree_exec.hip", "trees.cpp"]
 
 HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
 # -packed-fp32-ops off: hipcc must not emit packed FP32 instructions.
