@@ -63,6 +63,187 @@ def cpu_baseline(batch, physics, table, tol_abs, tol_rel, flags, target_seconds=
                       f"{wall:.2f} s wall"}
 
 
+MASS_HEADS = ("DISK_MASS_STELLAR", "SPHEROID_MASS_STELLAR", "DISK_MASS_GAS", "SPHEROID_MASS_GAS", "HOT_MASS", "BH_MASS")
+
+
+def _timed(fn, reps=2):
+    import torch
+    best = None
+    out = None
+    for _ in range(reps):
+        torch.cuda.synchronize()
+        t = time.perf_counter()
+        out = fn()
+        torch.cuda.synchronize()
+        dt = time.perf_counter() - t
+        best = dt if best is None else min(best, dt)
+    return best, out
+
+
+def surrogate_leg(physics, table, tol_abs, tol_rel, args):
+    """Surrogate training on integrator trajectories, then surrogate+fallback
+    against the plain integrator on held-out node intervals, and a chained
+    evolution of node histories from z=5 to z=0."""
+    import torch
+    from galacticus_amd import workloads as wl, surrogate as sg
+    from galacticus_amd.evolver import NodeEvolverBatched
+    n_hist = int(args.train_pairs / 12.5) + 1
+    ev = NodeEvolverBatched(max(n_hist, args.eval_pairs), physics, table, odeToleranceAbsolute=tol_abs,
+                            odeToleranceRelative=tol_rel)
+    t = time.perf_counter()
+    pairs = sg.generate_pairs(ev, wl.halo_histories(n_hist, seed=0, params=physics), args.train_pairs)
+    torch.cuda.synchronize()
+    gen_s = time.perf_counter() - t
+    log(f"surrogate: {pairs.n} training pairs in {gen_s:.2f} s")
+    mlp, norm, rep = sg.train_surrogate(ev, pairs, epochs=args.train_epochs, batch=8192)
+    log(f"surrogate: trained {rep.steps} steps in {rep.seconds:.2f} s ({rep.samples_per_s:.3g} samples/s, "
+        f"{rep.tflops:.2f} TFLOP/s), loss {rep.loss_first:.4g} -> {rep.loss_last:.4g}")
+    del pairs
+    test = sg.generate_pairs(ev, wl.halo_histories(int(args.eval_pairs / 12.5) + 1, seed=1, params=physics),
+                             args.eval_pairs)
+    n = test.n
+    gyr = float((test.t1 - test.t0).sum())
+    h0 = -torch.ones(n, dtype=torch.float64, device=test.t0.device)
+    holder = {}
+
+    def run_rk():
+        y, h = test.y0.clone(), h0.clone()
+        ev.evolve_device(y, test.node_par, test.t0, test.t1, h)
+        holder["rk"] = y
+    t_rk, _ = _timed(run_rk)
+    gate = sg.GateConfig()
+    se = sg.SurrogateEvolver(ev, mlp, norm, gate)
+
+    def run_sur():
+        y, h = test.y0.clone(), h0.clone()
+        st, acc, n_rej = se.evolve_device(y, test.node_par, test.t0, test.t1, h)
+        holder["sur"] = (y, acc, n_rej, st)
+    t_sur, _ = _timed(run_sur)
+    y_rk = holder["rk"]
+    y_s, acc, n_rej, st = holder["sur"]
+    V = wl.V
+    accb = acc.bool()
+    per_q = {}
+    for name in MASS_HEADS:
+        ref, got = y_rk[V[name]], y_s[V[name]]
+        sig = ref.abs() > 1e-6 * test.y0[V["HOT_MASS"]].abs().clamp_min(1.0)
+        m = accb & sig
+        rel = ((got - ref).abs() / ref.abs().clamp_min(1e-300))[m]
+        per_q[name] = {"median": float(rel.median()) if rel.numel() else 0.0,
+                       "p90": float(torch.quantile(rel[:2 ** 24].float(), 0.9)) if rel.numel() else 0.0}
+    ms_ref = y_rk[V["DISK_MASS_STELLAR"]] + y_rk[V["SPHEROID_MASS_STELLAR"]]
+    ms_sur = y_s[V["DISK_MASS_STELLAR"]] + y_s[V["SPHEROID_MASS_STELLAR"]]
+    ms_err = ((ms_sur - ms_ref).abs() / ms_ref.clamp_min(1e-300))
+    # the same surrogate on chained node histories from z=5 to z=0
+    chain = chained_histories(ev, se, physics, args)
+    rk_rate, sur_rate = gyr / t_rk, gyr / t_sur
+    log(f"surrogate: RK {rk_rate:.4g} node-Gyr/s, surrogate+fallback {sur_rate:.4g} ({sur_rate / rk_rate:.2f}x), "
+        f"rejected {n_rej / n:.3f}, median M* error {float(ms_err.median()):.2e}")
+    return {
+        "workload": "node-interval steps from node histories (simple timestep shortened by U(0.25,1)), "
+                    "FP64 integrator vs bf16 MLP + gate + FP64 fallback",
+        "training": {"pairs": int(rep.steps * rep.batch / max(rep.epochs, 1)), "epochs": rep.epochs, "batch": rep.batch,
+                     "steps": rep.steps, "seconds": rep.seconds, "samples_per_s": rep.samples_per_s,
+                     "data_generation_s": gen_s, "loss_first": rep.loss_first, "loss_last": rep.loss_last,
+                     "roofline": {"bound": "mfma", "achieved": rep.tflops, "peak": 2500.0, "unit": "TFLOP/s",
+                                  "frac": rep.tflops / 2500.0, "flop_per_sample": sg.FLOPS_TRAIN_PER_ROW}},
+        "inference": {"intervals": n, "node_gyr": gyr, "rk_node_gyr_per_s": rk_rate,
+                      "surrogate_node_gyr_per_s": sur_rate, "speedup": sur_rate / rk_rate,
+                      "rejected_fraction": n_rej / n, "statuses_ok": bool((st == 0).all()),
+                      "gate": dataclasses_asdict(gate),
+                      "median_stellar_mass_error_all": float(ms_err.median()),
+                      "accepted_relative_error": per_q},
+        "chained_to_z0": chain,
+    }
+
+
+def dataclasses_asdict(x):
+    import dataclasses
+    return dataclasses.asdict(x)
+
+
+def chained_histories(ev, se, physics, args, n=20000):
+    """Node histories from z=5 to z=0, interval by interval, once with the
+    integrator alone and once with the surrogate in front of it."""
+    import torch
+    from galacticus_amd import workloads as wl
+    hist = wl.halo_histories(n, seed=2, params=physics, start=float(wl.scale_factor_to_cosmic_time(1.0 / 6.0)))
+    b = hist.batch
+    dev = torch.device("cuda:0")
+    d = lambda a: torch.from_numpy(np.ascontiguousarray(a)).to(dev)
+    npar = d(b.node_par)
+    out = {}
+    times = {}
+    for mode in ("rk", "surrogate"):
+        rng = np.random.Generator(np.random.PCG64(99))
+        y, h = d(b.y), d(b.h_step)
+        t = b.t_start.copy()
+        n_rej = 0
+        n_int = 0
+        torch.cuda.synchronize()
+        t_wall = time.perf_counter()
+        while (t < hist.t_final - 1e-9).any():
+            t1 = hist.intervals(t, rng)
+            t0d, t1d = d(t), d(t1)
+            if mode == "rk":
+                ev.evolve_device(y, npar, t0d, t1d, h)
+            else:
+                _, _, r = se.evolve_device(y, npar, t0d, t1d, h)
+                n_rej += r
+            n_int += n
+            t = t1
+        torch.cuda.synchronize()
+        times[mode] = time.perf_counter() - t_wall
+        out[mode] = (y[wl.V["DISK_MASS_STELLAR"]] + y[wl.V["SPHEROID_MASS_STELLAR"]]).cpu().numpy()
+        if mode == "surrogate":
+            rej = n_rej / n_int
+    err = np.abs(out["surrogate"] - out["rk"]) / np.maximum(out["rk"], 1e-300)
+    return {"histories": n, "start": "z=5", "median_z0_stellar_mass_error": float(np.median(err)),
+            "p90_z0_stellar_mass_error": float(np.quantile(err, 0.9)), "rejected_fraction": rej,
+            "rk_seconds": times["rk"], "surrogate_seconds": times["surrogate"]}
+
+
+def trees_leg(physics, table, tol_abs, tol_rel, args):
+    """64 Parkinson-Cole-Helly trees (seed 0), roots log-uniform in 1e10-1e12
+    Msun at z=0, resolution 1e9 Msun, evolved from z=5 in the reference's walk
+    order, wave by wave."""
+    import torch
+    from galacticus_amd import trees as tr, workloads as wl
+    from galacticus_amd.evolver import NodeEvolverBatched
+    cosmo = tr.cosmology(physics)
+    t0 = float(wl.scale_factor_to_cosmic_time(1.0))
+    t_z5 = float(wl.scale_factor_to_cosmic_time(1.0 / 6.0))
+    rng = np.random.Generator(np.random.PCG64(0))
+    roots = 10.0 ** rng.uniform(10.0, 12.0, 64)
+    t = time.perf_counter()
+    forest = tr.Forest.build_pch(64, roots, t0, cosmo, tr.pch_params(1e9, t_z5), seed=0)
+    build_s = time.perf_counter() - t
+    wp = tr.walk_params(t0)
+    t = time.perf_counter()
+    sched = tr.Schedule(forest, cosmo, wp)
+    sched_s = time.perf_counter() - t
+    y0 = forest.initial_state(cosmo, wp)
+    ev = NodeEvolverBatched(forest.n_nodes, physics, table, odeToleranceAbsolute=tol_abs, odeToleranceRelative=tol_rel)
+    holder = {}
+
+    def run():
+        y = torch.from_numpy(y0).cuda()
+        holder["st"] = tr.evolve_forest(ev, sched, y)
+        holder["y"] = y
+    gpu_s, _ = _timed(run)
+    st = holder["st"].cpu().numpy()
+    gyr = sched.node_gyr()
+    k = np.bincount(sched.ops["kind"], minlength=4)
+    log(f"trees: {forest.n_nodes} nodes, {sched.n_ops} operations in {sched.n_waves} waves, GPU {gpu_s:.3f} s "
+        f"({gyr / gpu_s:.4g} node-Gyr/s)")
+    return {"workload": "64 PCH trees, seed 0, roots log-uniform 1e10-1e12 Msun, resolution 1e9 Msun, z=5 -> 0",
+            "nodes": forest.n_nodes, "operations": sched.n_ops, "waves": sched.n_waves,
+            "evolve_intervals": int(k[0]), "promotions": int(k[1]), "halo_mergers": int(k[2]),
+            "galaxy_mergers": int(k[3]), "node_gyr": gyr, "gpu_seconds": gpu_s, "node_gyr_per_s": gyr / gpu_s,
+            "host_build_seconds": build_s, "host_schedule_seconds": sched_s,
+            "statuses_ok": bool((st == 0).all())}
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -72,6 +253,11 @@ def main():
     ap.add_argument("--seed", type=int, default=0)
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
     ap.add_argument("--cpu-seconds", type=float, default=15.0)
+    ap.add_argument("--no-surrogate", action="store_true", help="skip the surrogate training/inference legs")
+    ap.add_argument("--no-trees", action="store_true", help="skip the merger-tree leg")
+    ap.add_argument("--train-pairs", type=int, default=4000000)
+    ap.add_argument("--train-epochs", type=int, default=20)
+    ap.add_argument("--eval-pairs", type=int, default=1000000)
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py measures one GPU: every box in this setup has one MI355X, and the batched evolver shards "
@@ -167,6 +353,11 @@ def main():
             "kernel_ms": kernel_ms,
         },
     }
+    del ys, hs
+    if not args.no_surrogate:
+        result["surrogate"] = surrogate_leg(physics, table, tol_abs, tol_rel, args)
+    if not args.no_trees:
+        result["trees"] = trees_leg(physics, table, tol_abs, tol_rel, args)
     if not args.no_cpu:
         cb = cpu_baseline(batch, physics, table, tol_abs, tol_rel, gp.FLAG_REUSE_STEP_SIZE, args.cpu_seconds)
         result["cpu_baseline"] = cb
