@@ -106,6 +106,8 @@ def surrogate_leg(physics, table, tol_abs, tol_rel, args):
     h0 = -torch.ones(n, dtype=torch.float64, device=test.t0.device)
     holder = {}
 
+    launch = ev.autotune(test.y0, test.node_par, test.t0, test.t1, h0)
+
     def run_rk():
         y, h = test.y0.clone(), h0.clone()
         ev.evolve_device(y, test.node_par, test.t0, test.t1, h)
@@ -147,7 +149,7 @@ def surrogate_leg(physics, table, tol_abs, tol_rel, args):
                      "data_generation_s": gen_s, "loss_first": rep.loss_first, "loss_last": rep.loss_last,
                      "roofline": {"bound": "mfma", "achieved": rep.tflops, "peak": 2500.0, "unit": "TFLOP/s",
                                   "frac": rep.tflops / 2500.0, "flop_per_sample": sg.FLOPS_TRAIN_PER_ROW}},
-        "inference": {"intervals": n, "node_gyr": gyr, "rk_node_gyr_per_s": rk_rate,
+        "inference": {"intervals": n, "node_gyr": gyr, "launch": list(launch), "rk_node_gyr_per_s": rk_rate,
                       "surrogate_node_gyr_per_s": sur_rate, "speedup": sur_rate / rk_rate,
                       "rejected_fraction": n_rej / n, "statuses_ok": bool((st == 0).all()),
                       "gate": dataclasses_asdict(gate),
@@ -285,6 +287,8 @@ def main():
     dev = torch.device("cuda:0")
     d = lambda a: torch.from_numpy(np.ascontiguousarray(a)).to(dev)
     y0, npar, t0, t1, h0 = d(batch.y), d(batch.node_par), d(batch.t_start), d(batch.t_end), d(batch.h_step)
+    launch = ev.autotune(y0, npar, t0, t1, h0)
+    log(f"autotuned launch: {launch[0]} nodes per workgroup, {launch[1]} workgroups per CU")
     total = args.warmup + args.steps
     ys = [y0.clone() for _ in range(total)]
     hs = [h0.clone() for _ in range(total)]
@@ -339,6 +343,7 @@ def main():
             "n_nodes": n, "n_vars": 30, "node_gyr": node_gyr, "seed": args.seed,
             "halo_mass": "log-uniform 1e9-1e13 Msun", "f_hot": "uniform 0.05-0.17", "metallicity": "log-uniform 1e-4-0.02",
             "dt_gyr": "uniform 0.1-2", "cooling_table": "200x50 FP64", "parallelism": "single GPU, one node per lane",
+            "launch": {"nodes_per_workgroup": launch[0], "workgroups_per_cu": launch[1]},
         },
         "roofline": {
             "bound": "mfma",

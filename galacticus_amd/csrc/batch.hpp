@@ -23,6 +23,8 @@ struct gal_ode_batch {
   double* d_params = nullptr;
   double* d_table = nullptr;
   unsigned int* d_work = nullptr;
+  double* d_ywork = nullptr;  // lane-private state/scales for two workgroups per CU
+  int ywork_lanes = 0;
   int* d_order = nullptr;
   int* d_index = nullptr;
   float* d_key = nullptr;

@@ -52,10 +52,7 @@ using namespace gal;
 namespace {
 
 constexpr int kStages = 5;  // k1..k5 kept; k6 is consumed as it is produced
-#ifndef GAL_SC_IN_LDS
-#define GAL_SC_IN_LDS 1
-#endif
-constexpr int kLdsVectors = kStages + 1 + GAL_SC_IN_LDS;
+constexpr int kLdsVectors = kStages + 2;  // k1..k5, y, scales
 
 struct EvolveArgs {
   int n, ld;
@@ -70,6 +67,8 @@ struct EvolveArgs {
   long long* counters;
   const int* order;
   unsigned int* work;
+  double* ywork;  // per-lane state and scales in HBM (variant without them in LDS)
+  int ystride;    // lanes in the grid (ywork holds 2*30 doubles per lane)
   const double* P;
   const double* tab;
   DerivedConstants D;
@@ -105,7 +104,12 @@ __device__ __forceinline__ void write_counters(const EvolveArgs& a, int node, lo
   }
 }
 
-template <int BLOCK>
+// YLDS: the state and the error scales of each lane live in LDS beside the
+// five stage derivatives (105 KiB per 64 lanes: one workgroup per CU).
+// Otherwise they live in a lane-private HBM workspace laid out like the LDS
+// (coalesced, cache-resident) and a workgroup needs 75 KiB of LDS, so two
+// workgroups share a CU and use two of its four SIMDs.
+template <int BLOCK, bool YLDS>
 __global__ __launch_bounds__(BLOCK, 1) void evolve_kernel(EvolveArgs a) {
   extern __shared__ double lds_k[];
   double* __restrict__ K = lds_k + threadIdx.x;   // K[(s*kNV + v)*BLOCK]
@@ -116,15 +120,14 @@ __global__ __launch_bounds__(BLOCK, 1) void evolve_kernel(EvolveArgs a) {
   const bool scales_given = (a.flags & GAL_FLAG_SCALES_PROVIDED) != 0;
 #define KS(s, v) K[((s) * kNV + (v)) * BLOCK]
 
-  double* __restrict__ YL = lds_k + kStages * kNV * BLOCK + threadIdx.x;
-#define Y(v) YL[(v) * BLOCK]
-#if GAL_SC_IN_LDS
-  double* __restrict__ SL = lds_k + (kStages + 1) * kNV * BLOCK + threadIdx.x;
-#define SC(v) SL[(v) * BLOCK]
-#else
-  double sc[kNV];
-#define SC(v) sc[v]
-#endif
+  // Both layouts are [vector][variable][lane] per workgroup: a wave reads
+  // 64 consecutive doubles per variable.
+  constexpr int ys = BLOCK;
+  double* __restrict__ YL = YLDS ? lds_k + kStages * kNV * BLOCK + threadIdx.x
+                                 : a.ywork + (size_t)blockIdx.x * (2 * kNV * BLOCK) + threadIdx.x;
+  double* __restrict__ SL = YL + kNV * BLOCK;
+#define Y(v) YL[(v) * ys]
+#define SC(v) SL[(v) * ys]
   double ytmp[kNV], kout[kNV];
   NodePar np{};
   int node = -1;
@@ -264,7 +267,7 @@ __global__ __launch_bounds__(BLOCK, 1) void evolve_kernel(EvolveArgs a) {
         for (int v = 0; v < kNV; ++v) KS(0, v) = kout[v];
         eval_pending = false;
         if (post_pending) {
-          post_step(P, a.D, np, mode, t, &Y(0), BLOCK);   // GSL_CONTINUE: no evolver reset
+          post_step(P, a.D, np, mode, t, &Y(0), ys);   // GSL_CONTINUE: no evolver reset
           post_pending = false;
         }
       }
@@ -303,7 +306,7 @@ __global__ __launch_bounds__(BLOCK, 1) void evolve_kernel(EvolveArgs a) {
         eval_pending = true;   // dydt at the new point, then the post-step
         post_pending = true;
       } else {
-        post_step(P, a.D, np, mode, t, &Y(0), BLOCK);
+        post_step(P, a.D, np, mode, t, &Y(0), ys);
         // odeSolverSolve returns xStep = driver h (solver.F90:593)
 #pragma unroll
         for (int v = 0; v < kNV; ++v) a.y[v * (size_t)a.ld + node] = Y(v);
@@ -435,9 +438,9 @@ __global__ void cost_probe_kernel(int n_items, const int* __restrict__ subset, i
   index[j] = i;
 }
 
-template <int BLOCK>
+template <int BLOCK, bool YLDS>
 constexpr size_t lds_bytes() {
-  return sizeof(double) * kLdsVectors * kNV * BLOCK;
+  return sizeof(double) * (YLDS ? kLdsVectors : kStages) * kNV * BLOCK;
 }
 
 }  // namespace
@@ -554,14 +557,21 @@ int gal_ode_batch_create(int n_max, double tol_abs, double tol_rel, int flags, g
     gal_ode_batch_destroy(h);
     return GAL_E_NOMEM;
   }
+  h->ywork_lanes = ncu * 2 * 64;
+  if (hipMalloc(&h->d_ywork, sizeof(double) * 2 * kNV * (size_t)h->ywork_lanes) != hipSuccess) {
+    gal_ode_batch_destroy(h);
+    return GAL_E_NOMEM;
+  }
   static bool attrs_set = false;
   if (!attrs_set) {
-    static_assert(lds_bytes<64>() <= 163840, "lane-private LDS vectors must fit 160 KiB");
-    if (hipFuncSetAttribute((const void*)evolve_kernel<64>, hipFuncAttributeMaxDynamicSharedMemorySize,
-                            (int)lds_bytes<64>()) != hipSuccess) {
+    static_assert(lds_bytes<64, true>() <= 163840, "lane-private LDS vectors must fit 160 KiB");
+    static_assert(2 * lds_bytes<64, false>() <= 163840, "two workgroups per CU must fit 160 KiB");
+    if (hipFuncSetAttribute((const void*)evolve_kernel<64, true>, hipFuncAttributeMaxDynamicSharedMemorySize,
+                            (int)lds_bytes<64, true>()) != hipSuccess ||
+        hipFuncSetAttribute((const void*)evolve_kernel<64, false>, hipFuncAttributeMaxDynamicSharedMemorySize,
+                            (int)lds_bytes<64, false>()) != hipSuccess) {
       (void)hipGetLastError();
-      hipFree(h->d_params); hipFree(h->d_table); hipFree(h->d_work); hipFree(h->d_order);
-      delete h;
+      gal_ode_batch_destroy(h);
       return GAL_E_HIP;
     }
     attrs_set = true;
@@ -583,6 +593,7 @@ int gal_ode_batch_destroy(gal_ode_batch* h) {
   hipFree(h->d_params);
   hipFree(h->d_table);
   hipFree(h->d_work);
+  hipFree(h->d_ywork);
   hipFree(h->d_order);
   hipFree(h->d_index);
   hipFree(h->d_key);
@@ -635,9 +646,9 @@ int gal_ode_batch_set_cooling_table(gal_ode_batch* h, const double* ln_lambda, i
 int gal_ode_batch_set_launch(gal_ode_batch* h, int nodes_per_block, int blocks_per_cu) {
   if (!h) return GAL_E_INVALID;
   if (nodes_per_block != 64) return GAL_E_INVALID;
-  // the stage derivatives of all lanes of a CU must fit its 160 KiB of LDS
-  if (blocks_per_cu < 1 || (size_t)blocks_per_cu * sizeof(double) * kLdsVectors * kNV * nodes_per_block > 163840)
-    return GAL_E_INVALID;
+  // one workgroup per CU keeps state and scales in LDS; two keep them in the
+  // HBM workspace; the stage derivatives of a CU must fit its 160 KiB of LDS
+  if (blocks_per_cu < 1 || blocks_per_cu > 2) return GAL_E_INVALID;
   h->nodes_per_block = nodes_per_block;
   h->blocks_per_cu = blocks_per_cu;
   return GAL_OK;
@@ -646,7 +657,16 @@ int gal_ode_batch_set_launch(gal_ode_batch* h, int nodes_per_block, int blocks_p
 static int launch_evolve(gal_ode_batch* h, const EvolveArgs& args, hipStream_t s) {
   const int blocks = std::max(1, h->n_cus * h->blocks_per_cu);
   hipLaunchKernelGGL(reset_counter_kernel, dim3(1), dim3(64), 0, s, args.work);
-  hipLaunchKernelGGL(evolve_kernel<64>, dim3(blocks), dim3(64), lds_bytes<64>(), s, args);
+  constexpr size_t lds_y = lds_bytes<64, true>(), lds_k = lds_bytes<64, false>();
+  if (h->blocks_per_cu == 1) {
+    hipLaunchKernelGGL((evolve_kernel<64, true>), dim3(blocks), dim3(64), lds_y, s, args);
+  } else {
+    EvolveArgs b = args;
+    b.ywork = h->d_ywork;
+    b.ystride = blocks * 64;
+    if (b.ystride > h->ywork_lanes) return GAL_E_INVALID;
+    hipLaunchKernelGGL((evolve_kernel<64, false>), dim3(blocks), dim3(64), lds_k, s, b);
+  }
   HIP_TRY(hipGetLastError());
   return GAL_OK;
 }
@@ -671,6 +691,8 @@ static int make_args(gal_ode_batch* h, int n, int ld, double* y, const double* n
   a.counters = reinterpret_cast<long long*>(counters);
   a.order = nullptr;
   a.work = h->d_work;
+  a.ywork = nullptr;
+  a.ystride = 0;
   a.P = h->d_params;
   a.tab = h->d_table;
   a.D = h->D;

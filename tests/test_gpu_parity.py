@@ -190,3 +190,23 @@ def test_evolve_matches_oracle(torch_cuda, cooling_table, reuse):
     assert med < 1e-9
     assert same_steps > 0.95
     np.testing.assert_allclose(hg[cntg[0] == cnto[0]], ho[cntg[0] == cnto[0]], rtol=1e-6)
+
+
+def test_launch_variants_bit_identical(torch_cuda, cooling_table):
+    """One workgroup per CU (state in LDS) and two per CU (state in the HBM
+    workspace) run the same arithmetic: identical results, bit for bit."""
+    torch = torch_cuda
+    physics = gp.physics_params()
+    b = wl.isolated_halos(4000, seed=21, params=physics)
+    ev = _evolver(b.n, physics)
+    d = lambda a: torch.from_numpy(np.ascontiguousarray(a)).cuda()
+    out = []
+    for shape in ((64, 1), (64, 2)):
+        ev.set_launch(*shape)
+        y, h = d(b.y), d(b.h_step)
+        st = ev.evolve_device(y, d(b.node_par), d(b.t_start), d(b.t_end), h)
+        torch.cuda.synchronize()
+        out.append((y.cpu(), h.cpu(), st.cpu()))
+    assert torch.equal(out[0][0], out[1][0]) and torch.equal(out[0][1], out[1][1]) and torch.equal(out[0][2], out[1][2])
+    shape = ev.autotune(d(b.y), d(b.node_par), d(b.t_start), d(b.t_end), d(b.h_step))
+    assert shape in ((64, 1), (64, 2))
