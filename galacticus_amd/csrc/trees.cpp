@@ -9,7 +9,10 @@
 #include <limits>
 #include <memory>
 #include <new>
+#include <atomic>
+#include <cstdlib>
 #include <random>
+#include <thread>
 #include <vector>
 
 #include "../../include/gal_tree.h"
@@ -682,6 +685,35 @@ struct Walk {
   }
 };
 
+// Run fn(i) for i in [0, n) on the host's cores (trees are independent);
+// OMP_NUM_THREADS, when set, caps the number of threads.
+template <typename F>
+void parallel_trees(int n, F fn) {
+  int nt = (int)std::thread::hardware_concurrency();
+  if (const char* e = std::getenv("OMP_NUM_THREADS")) {
+    const int v = std::atoi(e);
+    if (v > 0) nt = std::min(nt, v);
+  }
+  nt = std::max(1, std::min(nt, n));
+  std::atomic<int> next{0};
+  std::atomic<bool> failed{false};
+  auto work = [&]() {
+    try {
+      for (int i = next++; i < n; i = next++) fn(i);
+    } catch (...) {
+      failed = true;
+    }
+  };
+  if (nt == 1) {
+    work();
+  } else {
+    std::vector<std::thread> pool;
+    for (int k = 0; k < nt; ++k) pool.emplace_back(work);
+    for (auto& th : pool) th.join();
+  }
+  if (failed) throw std::bad_alloc();
+}
+
 }  // namespace
 
 struct gal_schedule {
@@ -717,11 +749,27 @@ int gal_forest_build_pch(const gal_cosmology* cosmo, const gal_pch_params* pch, 
     double m_max = 0.0;
     for (int i = 0; i < n_trees; ++i) m_max = std::max(m_max, root_mass[i]);
     Branching br(c, *pch, m_max);
+    // each tree from its own generator, built in parallel, then concatenated in order
+    std::vector<gal_forest> parts(n_trees);
+    parallel_trees(n_trees, [&](int i) {
+      std::mt19937_64 rng(seed * 0x9E3779B97F4A7C15ull + (uint64_t)i * 0xD1B54A32D192ED03ull + 1ull);
+      build_tree(c, br, root_mass[i], time_root, rng, parts[i]);
+    });
     auto f = std::make_unique<gal_forest>();
     f->tree_offset.push_back(0);
-    for (int i = 0; i < n_trees; ++i) {
-      std::mt19937_64 rng(seed * 0x9E3779B97F4A7C15ull + (uint64_t)i * 0xD1B54A32D192ED03ull + 1ull);
-      build_tree(c, br, root_mass[i], time_root, rng, *f);
+    for (gal_forest& p : parts) {
+      const int32_t base = (int32_t)f->parent.size();
+      auto shift = [&](int32_t k) { return k < 0 ? -1 : k + base; };
+      for (size_t k = 0; k < p.parent.size(); ++k) {
+        f->parent.push_back(shift(p.parent[k]));
+        f->first_child.push_back(shift(p.first_child[k]));
+        f->sibling.push_back(shift(p.sibling[k]));
+      }
+      f->mass.insert(f->mass.end(), p.mass.begin(), p.mass.end());
+      f->time.insert(f->time.end(), p.time.begin(), p.time.end());
+      f->tree_offset.push_back((int64_t)f->parent.size());
+      f->n_trees += 1;
+      p = gal_forest();
     }
     *out = f.release();
   } catch (const std::bad_alloc&) {
@@ -789,9 +837,17 @@ int gal_schedule_build(const gal_forest* f, const gal_cosmology* cosmo, const ga
   try {
     Cosmology c(*cosmo);
     auto s = std::make_unique<gal_schedule>();
-    for (int t = 0; t < f->n_trees; ++t) {
-      Walk wk(*f, c, *walk, s->ops, f->tree_offset[t], f->tree_offset[t + 1]);
-      if (!wk.run()) return GAL_E_INVALID;
+    std::vector<std::vector<gal_schedule_op>> parts(f->n_trees);
+    std::vector<char> ok(f->n_trees, 1);
+    parallel_trees(f->n_trees, [&](int t) {
+      Walk wk(*f, c, *walk, parts[t], f->tree_offset[t], f->tree_offset[t + 1]);
+      ok[t] = wk.run() ? 1 : 0;
+    });
+    for (int t = 0; t < f->n_trees; ++t)
+      if (!ok[t]) return GAL_E_INVALID;
+    for (auto& p : parts) {
+      s->ops.insert(s->ops.end(), p.begin(), p.end());
+      std::vector<gal_schedule_op>().swap(p);
     }
     // waves: one more than the latest earlier operation on the same node(s)
     std::vector<int32_t> last(f->parent.size(), -1);
