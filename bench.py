@@ -253,6 +253,8 @@ def main():
     ap.add_argument("--warmup", type=int, default=2)
     ap.add_argument("--nodes", type=int, default=100000)
     ap.add_argument("--seed", type=int, default=0)
+    ap.add_argument("--streams", type=int, default=2,
+                    help="independent batches alternate over this many streams and evolver handles")
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
     ap.add_argument("--cpu-seconds", type=float, default=15.0)
     ap.add_argument("--no-surrogate", action="store_true", help="skip the surrogate training/inference legs")
@@ -260,6 +262,8 @@ def main():
     ap.add_argument("--train-pairs", type=int, default=4000000)
     ap.add_argument("--train-epochs", type=int, default=20)
     ap.add_argument("--eval-pairs", type=int, default=1000000)
+    ap.add_argument("--copy-calibration-mib", type=int, default=0,
+                    help="run one elementwise copy of this many MiB first (reference kernel for HBM counters)")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py measures one GPU: every box in this setup has one MI355X, and the batched evolver shards "
@@ -282,23 +286,45 @@ def main():
     batch = wl.isolated_halos(args.nodes, seed=args.seed, params=physics)
     n = batch.n
     node_gyr = batch.node_gyr
-    ev = NodeEvolverBatched(n, physics, table, odeToleranceAbsolute=tol_abs, odeToleranceRelative=tol_rel,
-                            reuseODEStepSize=True)
+    n_streams = max(1, args.streams)
+    evs = [NodeEvolverBatched(n, physics, table, odeToleranceAbsolute=tol_abs, odeToleranceRelative=tol_rel,
+                              reuseODEStepSize=True) for _ in range(n_streams)]
+    ev = evs[0]
     dev = torch.device("cuda:0")
     d = lambda a: torch.from_numpy(np.ascontiguousarray(a)).to(dev)
     y0, npar, t0, t1, h0 = d(batch.y), d(batch.node_par), d(batch.t_start), d(batch.t_end), d(batch.h_step)
+    if args.copy_calibration_mib > 0:
+        n_el = args.copy_calibration_mib * (1 << 20) // 8
+        src = torch.ones(n_el, dtype=torch.float64, device=dev)
+        dst = torch.empty_like(src)
+        torch.mul(src, 1.0, out=dst)          # reads and writes n_el * 8 bytes
+        torch.cuda.synchronize()
+        log(f"copy calibration: {n_el * 8} bytes read and written by one elementwise kernel")
+        del src, dst
     launch = ev.autotune(y0, npar, t0, t1, h0)
+    for e in evs[1:]:
+        e.set_launch(*launch)
     log(f"autotuned launch: {launch[0]} nodes per workgroup, {launch[1]} workgroups per CU")
     total = args.warmup + args.steps
     ys = [y0.clone() for _ in range(total)]
     hs = [h0.clone() for _ in range(total)]
-    scale = torch.empty_like(y0)
-    status = torch.empty(n, dtype=torch.int32, device=dev)
-    counters = torch.zeros(3 * n, dtype=torch.int64, device=dev)
+    # Consecutive steps are independent batches: they alternate over n_streams
+    # evolver handles and streams, so one batch's tail (the few stiffest
+    # nodes) overlaps the next batch's start.
+    streams = [torch.cuda.Stream() for _ in range(n_streams)]
+    scales = [torch.empty_like(y0) for _ in range(n_streams)]
+    statuses = [torch.empty(n, dtype=torch.int32, device=dev) for _ in range(n_streams)]
+    counters = [torch.zeros(3 * n, dtype=torch.int64, device=dev) for _ in range(n_streams)]
     torch.cuda.synchronize()
 
+    def launch_step(i, k):
+        j = k % n_streams
+        evs[j].evolve_device(ys[i], npar, t0, t1, hs[i], scale=scales[j], status=statuses[j], counters=counters[j],
+                             stream=streams[j])
+        return j
+
     for i in range(args.warmup):
-        ev.evolve_device(ys[i], npar, t0, t1, hs[i], scale=scale, status=status, counters=counters)
+        launch_step(i, i)
     torch.cuda.synchronize()
     log(f"warmup done ({args.warmup} steps)")
 
@@ -308,22 +334,26 @@ def main():
     t_wall = time.perf_counter()
     for k in range(args.steps):
         i = args.warmup + k
-        starts[k].record()
-        ev.evolve_device(ys[i], npar, t0, t1, hs[i], scale=scale, status=status, counters=counters)
-        ends[k].record()
+        j = k % n_streams
+        starts[k].record(streams[j])
+        launch_step(i, k)
+        ends[k].record(streams[j])
     torch.cuda.synchronize()
     wall = time.perf_counter() - t_wall
     kernel_ms = float(np.mean([s.elapsed_time(e) for s, e in zip(starts, ends)]))
     ms_per_step = wall * 1e3 / args.steps
-    st = status.cpu().numpy()
-    if not (st == 0).all():
-        log(f"non-success statuses: {np.unique(st, return_counts=True)}")
-        sys.exit(1)
-    cnt = counters.view(3, n).cpu().numpy()
+    for st in statuses:
+        st = st.cpu().numpy()
+        if not (st == 0).all():
+            log(f"non-success statuses: {np.unique(st, return_counts=True)}")
+            sys.exit(1)
+    cnt = counters[0].view(3, n).cpu().numpy()
     attempts = int(cnt[0].sum())
     flop_step = gflops.attempt_flops()
     achieved_tflops = attempts * flop_step / (kernel_ms * 1e-3) / 1e12
-    log(f"step {ms_per_step:.3f} ms (kernel {kernel_ms:.3f} ms), attempts {attempts}, max/node {cnt[0].max()}")
+    aggregate_tflops = attempts * args.steps * flop_step / wall / 1e12
+    log(f"step {ms_per_step:.3f} ms (one launch {kernel_ms:.3f} ms, {n_streams} streams), attempts {attempts}, "
+        f"max/node {cnt[0].max()}")
 
     result = {
         "metric": METRIC,
@@ -343,7 +373,7 @@ def main():
             "n_nodes": n, "n_vars": 30, "node_gyr": node_gyr, "seed": args.seed,
             "halo_mass": "log-uniform 1e9-1e13 Msun", "f_hot": "uniform 0.05-0.17", "metallicity": "log-uniform 1e-4-0.02",
             "dt_gyr": "uniform 0.1-2", "cooling_table": "200x50 FP64", "parallelism": "single GPU, one node per lane",
-            "launch": {"nodes_per_workgroup": launch[0], "workgroups_per_cu": launch[1]},
+            "launch": {"nodes_per_workgroup": launch[0], "workgroups_per_cu": launch[1], "streams": n_streams},
         },
         "roofline": {
             "bound": "mfma",
@@ -356,6 +386,7 @@ def main():
             "flop_per_attempt": flop_step,
             "attempts_per_launch": attempts,
             "kernel_ms": kernel_ms,
+            "aggregate_tflops": aggregate_tflops,
         },
     }
     del ys, hs

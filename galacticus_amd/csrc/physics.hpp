@@ -158,7 +158,8 @@ __device__ __forceinline__ void isco(double a, double& energy, double& ang_mom) 
   const double cp = cbrt(1.0 + a), cm = cbrt(1.0 - a);
   const double a1 = 1.0 + cp * cm * (cp + cm);
   const double a2 = sqrt(3.0 * a * a + a1 * a1);
-  const double r = 3.0 + a2 - sqrt((3.0 - a1) * (3.0 + a1 + 2.0 * a2));
+  // A1 <= 3 for |a| <= 1; rounding can push it just above 3 for tiny spins
+  const double r = 3.0 + a2 - sqrt(fmax(3.0 - a1, 0.0) * (3.0 + a1 + 2.0 * a2));
   const double sr = sqrt(r);
   const double inv_den = rcp(r) * rsqrt(r * r - 3.0 * r + 2.0 * a * sr);
   energy = (r * r - 2.0 * r + a * sr) * inv_den;

@@ -437,7 +437,10 @@ static double isco_radius(double a)
 {
   double a1 = 1.0 + cbrt(1.0 - a * a) * (cbrt(1.0 + a) + cbrt(1.0 - a));
   double a2 = sqrt(3.0 * a * a + a1 * a1);
-  return 3.0 + a2 - sqrt((3.0 - a1) * (3.0 + a1 + 2.0 * a2));
+  /* A1 <= 3 for |a| <= 1; rounding can push it just above 3 for tiny spins */
+  double d = 3.0 - a1;
+  if (d < 0.0) d = 0.0;
+  return 3.0 + a2 - sqrt(d * (3.0 + a1 + 2.0 * a2));
 }
 static double isco_energy(double a)
 {

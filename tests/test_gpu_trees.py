@@ -6,9 +6,10 @@
   (testSuite/test-reproducibility.py:47-57) at the root.
 * A Parkinson-Cole-Helly forest run wave by wave on the GPU equals the same
   schedule applied one operation at a time on the CPU with the oracle
-  integrator and the oracle's merger mass movements (oracle/trees.py): every
-  node's final state within the ODE tolerance, median relative difference
-  below 1e-6, identical statuses.
+  integrator and the oracle's merger mass movements (oracle/trees.py): median
+  relative difference below 1e-9 (both sides take the same steps almost
+  everywhere), every mass within 50x the per-interval ODE tolerance where
+  they do not, identical statuses.
 """
 import math
 import os
@@ -94,8 +95,9 @@ def test_forest_waves_match_sequential_oracle(torch_cuda, cooling_table):
     ev = NodeEvolverBatched(forest.n_nodes, physics, cooling_table, odeToleranceAbsolute=1e-6,
                             odeToleranceRelative=1e-3)
     yd = torch.from_numpy(y0.copy()).cuda()
-    st = tr.evolve_forest(ev, sched, yd).cpu().numpy()
-    yg = yd.cpu().numpy()
+    sd = torch.zeros_like(yd)
+    st = tr.evolve_forest(ev, sched, yd, scale=sd).cpu().numpy()
+    yg, sc = yd.cpu().numpy(), sd.cpu().numpy()
     yo, _, sto = _oracle_apply(sched, y0, physics, cooling_table, 1e-6, 1e-3)
     evolved = np.unique(sched.ops["a"][sched.ops["kind"] == tr.OP_EVOLVE])
     assert (st[evolved] == 0).all() and (sto[evolved] == 0).all()
@@ -105,8 +107,20 @@ def test_forest_waves_match_sequential_oracle(torch_cuda, cooling_table):
     big = ref > 1e-6 * np.abs(yo[V["HOT_MASS"]]).max()
     rel = np.abs(yg[mass] - yo[mass])[big] / ref[big]
     print("median rel", np.median(rel), "max rel", rel.max(), flush=True)
-    assert np.median(rel) < 1e-6
-    assert rel.max() < 2e-2
+    assert np.median(rel) < 1e-9                      # the same steps on both sides, almost everywhere
+    # where the two integrations took different steps they agree to the ODE
+    # tolerance per interval, accumulated over each node's history
+    bad = ~(np.isfinite(yg) & np.isfinite(yo) & np.isfinite(sc))
+    if bad.any():
+        v, i = np.nonzero(bad)
+        for vv, ii in list(zip(v, i))[:10]:
+            print("non-finite", wl.VAR_NAMES[vv], "node", ii, "gpu", yg[vv, ii], "oracle", yo[vv, ii], "scale",
+                  sc[vv, ii], "ops", [tuple(o)[:6] for o in sched.ops if o["a"] == ii or o["b"] == ii][-4:], flush=True)
+    assert not bad.any()
+    tol = 1e-3 * np.abs(yo[mass]) + 1e-6 * np.abs(sc[mass])
+    worst = np.abs(yg[mass] - yo[mass]) / np.maximum(tol, 1e-300)
+    print("max |dy| / tolerance", worst.max(), flush=True)
+    assert worst.max() < 50.0
     roots = forest.roots()
     mstar_g = yg[V["DISK_MASS_STELLAR"], roots] + yg[V["SPHEROID_MASS_STELLAR"], roots]
     mstar_o = yo[V["DISK_MASS_STELLAR"], roots] + yo[V["SPHEROID_MASS_STELLAR"], roots]

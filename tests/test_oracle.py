@@ -342,3 +342,23 @@ def test_underflow_status_when_tolerance_impossible(table):
                                       gp.FLAG_REUSE_STEP_SIZE)
     assert (st == 15).all()
     assert np.array_equal(y, b.y)
+
+
+def test_black_hole_spin_near_zero_stays_finite(cooling_table):
+    """Regression: for spins of order 1e-10 the ISCO factor A1 rounds to just
+    above 3 and sqrt(3 - A1) turned NaN.  A leaf halo of a tree (hot gas only,
+    a 100 Msun black hole with zero spin) spins its black hole up through that
+    range in its first interval."""
+    from galacticus_amd import params as gp, trees as tr, workloads as wl
+    p = gp.physics_params()
+    cosmo = tr.cosmology(p)
+    t0 = float(wl.scale_factor_to_cosmic_time(1.0))
+    t_z5 = float(wl.scale_factor_to_cosmic_time(1.0 / 6.0))
+    forest = tr.Forest.build_pch(8, np.geomspace(1e11, 1e12, 8), t0, cosmo, tr.pch_params(2e10, t_z5), seed=2)
+    wp = tr.walk_params(t0, merging_multiplier=0.2)
+    op = tr.Schedule(forest, cosmo, wp).ops[0]
+    y0 = forest.initial_state(cosmo, wp)[:, [op["a"]]]
+    y, h, sc, st, _ = oracle.evolve(y0, np.asarray(op["node_par"])[:, None], [op["t0"]], [op["t1"]], [-1.0], p,
+                                    cooling_table, 1e-6, 1e-3, gp.FLAG_REUSE_STEP_SIZE, root_tol_rel=1e-15)
+    assert np.isfinite(y).all() and (st == 0).all()
+    assert y[wl.V["BH_SPIN"], 0] > 0.0
