@@ -193,6 +193,9 @@ __global__ __launch_bounds__(256) void loss_kernel(LossArgs a) {
       d[v] = f2bf(2.0f * a.w[v] * res * a.inv_wsum * a.inv_B);
       if ((a.head_mask >> v) & 1u) emax = fmaxf(emax, fabsf(res) * a.sigma[v]);
     }
+    // a sample whose target was clipped carries a flag in column 31: its true
+    // error is at least 1 (asinh units), so the head learns to reject it
+    if (t[31] != 0.0f) emax = fmaxf(emax, 1.0f);
     const float rh = o[GAL_MLP_HEAD] - logf(emax + 1.0e-7f);
     l = mse * a.inv_wsum + a.head_w * rh * rh;
     d[GAL_MLP_HEAD] = f2bf(2.0f * a.head_w * rh * a.inv_B);
@@ -388,9 +391,12 @@ __device__ __forceinline__ double var_scale(const NodeFrame& f, int role) {
   }
 }
 
-// The first-order (explicit Euler) increment of each trained variable in
-// asinh space, from one RHS evaluation at t0.  The network learns only what
-// this misses.
+// The first-order increment of each trained variable in asinh space from one
+// RHS evaluation at t0: for a growing quantity the Euler step taken in y,
+// asinh((y + f dt)/s) - asinh(y/s) (exact for a constant rate); for a
+// decaying one the Euler step taken in u, f dt / sqrt(s^2 + y^2) (exact for
+// exponential decay, and never overshoots through zero).  The network
+// learns only what this misses.
 __device__ __forceinline__ void euler_baseline(const double* __restrict__ P, const DerivedConstants& D,
                                                const double* __restrict__ tab, const CoolingDesc& cd, int mode,
                                                const NodePar& np, const NodeFrame& f, double t0, double dt,
@@ -405,7 +411,8 @@ __device__ __forceinline__ void euler_baseline(const double* __restrict__ P, con
       d = asinh(rate[v] * dt / f.s_energy);
     } else if (role != R_CONST && role != R_SPIN) {
       const double s = var_scale(f, role);
-      d = asinh((y0[v] + rate[v] * dt) / s) - asinh(y0[v] / s);
+      d = rate[v] >= 0.0 ? asinh((y0[v] + rate[v] * dt) / s) - asinh(y0[v] / s)
+                         : rate[v] * dt * rsqrt(s * s + y0[v] * y0[v]);
     }
     du[v] = d;
   }

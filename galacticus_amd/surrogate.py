@@ -39,9 +39,9 @@ V = wl.V
 INPUT_VARS = [0, 1, 2, 3, 4, 5, 6, 7, 12, 13, 15, 16, 17, 18, 19, 20, 21, 22, 23, 24, 25, 26]
 TRAINED_VARS = sorted(INPUT_VARS + [V["HOT_ENERGY_RADIATED"]])
 # The error head predicts the worst asinh-space error of these masses
-# (stellar, cold gas, hot gas and black hole).
-HEAD_VARS = [V["DISK_MASS_STELLAR"], V["SPHEROID_MASS_STELLAR"], V["DISK_MASS_GAS"], V["SPHEROID_MASS_GAS"],
-             V["HOT_MASS"], V["BH_MASS"]]
+# (stellar, hot gas and black hole); cold-gas errors are reported, and the
+# baryon and metal budgets of the gate bound them.
+HEAD_VARS = [V["DISK_MASS_STELLAR"], V["SPHEROID_MASS_STELLAR"], V["HOT_MASS"], V["BH_MASS"]]
 
 FLOPS_FORWARD_PER_ROW = 2 * sum(a * b for a, b in zip(FAN_IN, FAN_OUT))
 FLOPS_TRAIN_PER_ROW = 3 * FLOPS_FORWARD_PER_ROW
@@ -245,20 +245,33 @@ def encode(evolver: NodeEvolverBatched, pairs: PairSet, norm=None, raw=False, wi
     return x, x_raw, tg
 
 
-def fit_normalization(x_raw, t_raw):
-    """Standardisation (4 x 32 fp32): input mean/std, target mean/std."""
+def fit_normalization(x_raw, t_raw, sample=2_000_000):
+    """Standardisation (4 x 32 fp32): input and target means and standard
+    deviations (from at most `sample` rows)."""
     import torch
+    xs = x_raw[:sample].double()
+    ts = t_raw[:sample].double()
     norm = torch.zeros((4, 32), dtype=torch.float32, device=x_raw.device)
-    norm[0] = x_raw.double().mean(0).float()
-    norm[1] = x_raw.double().std(0).float()
-    norm[2] = t_raw.double().mean(0).float()
-    norm[3] = t_raw.double().std(0).float()
+    norm[0] = xs.mean(0).float()
+    norm[1] = xs.std(0).float()
+    norm[2] = ts.mean(0).float()
+    norm[3] = ts.std(0).float()
     norm[1] = torch.where(norm[1] > 1e-6, norm[1], torch.ones_like(norm[1]))
     untrained = [v for v in range(32) if v not in TRAINED_VARS]
     norm[2, untrained] = 0.0
     norm[3, untrained] = 1.0
     norm[3] = torch.where(norm[3] > 1e-9, norm[3], torch.ones_like(norm[3]))
     return norm.contiguous()
+
+
+def clip_targets(target, clip=8.0):
+    """Clip standardised targets to +-clip and flag clipped rows in column 31
+    (their error counts as large for the error head)."""
+    import torch
+    big = (target[:, :30].abs() > clip).any(dim=1)
+    target[:, :30].clamp_(-clip, clip)
+    target[:, 31] = big.to(target.dtype)
+    return float(big.float().mean())
 
 
 @dataclasses.dataclass
@@ -272,6 +285,7 @@ class TrainReport:
     loss_first: float
     loss_last: float
     losses: list
+    clipped_fraction: float = 0.0
 
 
 def train_surrogate(evolver: NodeEvolverBatched, pairs: PairSet, epochs: int = 20, batch: int = 8192,
@@ -284,6 +298,7 @@ def train_surrogate(evolver: NodeEvolverBatched, pairs: PairSet, epochs: int = 2
     norm = fit_normalization(x_raw, t_raw)
     del x_raw, t_raw
     x, _, target = encode(evolver, pairs, norm=norm)
+    clipped = clip_targets(target)
     n = pairs.n
     steps_per_epoch = n // batch
     if steps_per_epoch < 1:
@@ -311,7 +326,7 @@ def train_surrogate(evolver: NodeEvolverBatched, pairs: PairSet, epochs: int = 2
     steps = epochs * steps_per_epoch
     rep = TrainReport(epochs, steps, batch, secs, steps * batch / secs, steps * batch * FLOPS_TRAIN_PER_ROW / secs / 1e12,
                       float(trace[:10].mean()), float(trace[-10:].mean()),
-                      [float(v) for v in trace[::max(1, steps // 50)]])
+                      [float(v) for v in trace[::max(1, steps // 50)]], clipped)
     return mlp, norm, rep
 
 

@@ -26,9 +26,12 @@
  *   25 Mdot_halo dt / M_halo; 26 ln spin parameter; 27 r_outer / r_vir;
  *   28 V_vir/200; 29 reionization suppression flag; 30 ln(dt / t_dyn); 31 0.
  *   Targets 0..29: u_v(t1) - u_v(t0) with the t0 scales (the radiated
- *   energy uses asinh(dE / s_E)), minus the same quantity for the explicit
- *   Euler step y0 + f(t0, y0) dt (one RHS evaluation): the network learns
- *   the residual of a first-order step.  Column 30 is the error head.
+ *   energy uses asinh(dE / s_E)), minus a first-order increment from one
+ *   RHS evaluation f = f(t0, y0): asinh((y + f dt)/s) - asinh(y/s) where
+ *   f >= 0, f dt / sqrt(s^2 + y^2) where f < 0, asinh(f dt / s_E) for the
+ *   energy: the network learns the residual of a first-order step.  Column 30 is the error head; a nonzero column 31
+ *   marks a training sample whose targets were clipped (its error counts
+ *   as at least 1 in the head's loss).
  *   Variables with zero rate are carried over, the spin angular momentum is
  *   advanced analytically (its rate is the node's constant spin rate).
  * Normalisation array norm (device fp32, 4 x 32):
@@ -85,7 +88,8 @@ int gal_mlp_set_adam(gal_mlp* m, double lr, double lr_min, double beta1, double 
 /* Loss per row: sum_v w_v r_v^2 / sum_v w_v + head_weight (o_30 - ln(e + 1e-7))^2
    with r_v = o_v - target_v (standardised) for v < 30 and
    e = max over head_vars of |r_v| * target_sigma_v (the step's error in
-   asinh units).  var_weight, target_sigma, head_vars: host arrays of 30. */
+   asinh units), raised to at least 1 when target column 31 is nonzero.
+   var_weight, target_sigma, head_vars: host arrays of 30. */
 int gal_mlp_set_loss(gal_mlp* m, const float* var_weight, const float* target_sigma, const int* head_vars,
                      double head_weight);
 

@@ -102,6 +102,7 @@ def loss_grad(out, target, w, sigma, head_mask, head_w):
     mse = (w * r * r).sum(axis=1)
     hm = np.asarray(head_mask, dtype=bool)
     e = (np.abs(r[:, hm]) * np.asarray(sigma, dtype=np.float32)[hm]).max(axis=1)
+    e = np.where(target[:, 31] != 0, np.maximum(e, np.float32(1.0)), e)   # clipped samples
     rh = out[:, HEAD] - np.log(e + np.float32(1e-7)).astype(np.float32)
     loss = float((mse / wsum + np.float32(head_w) * rh * rh).sum() / n)
     g = np.zeros((n, OUT_LD), dtype=np.float32)
@@ -147,7 +148,11 @@ def loss64(out, target, w, sigma, head_mask, head_w, e_fixed=None):
     r = out[:, :30] - target[:, :30]
     mse = (w * r * r).sum(axis=1) / w.sum()
     hm = np.asarray(head_mask, dtype=bool)
-    e = (np.abs(r[:, hm]) * np.asarray(sigma)[hm]).max(axis=1) if e_fixed is None else e_fixed
+    if e_fixed is None:
+        e = (np.abs(r[:, hm]) * np.asarray(sigma)[hm]).max(axis=1)
+        e = np.where(target[:, 31] != 0, np.maximum(e, 1.0), e)          # clipped samples
+    else:
+        e = e_fixed
     rh = out[:, HEAD] - np.log(e + 1e-7)
     return float((mse + head_w * rh * rh).sum() / n), e
 
@@ -158,6 +163,7 @@ def backward64(p, cache, out, target, w, sigma, head_mask, head_w):
     r = out[:, :30] - target[:, :30]
     hm = np.asarray(head_mask, dtype=bool)
     e = (np.abs(r[:, hm]) * np.asarray(sigma)[hm]).max(axis=1)
+    e = np.where(target[:, 31] != 0, np.maximum(e, 1.0), e)              # clipped samples
     rh = out[:, HEAD] - np.log(e + 1e-7)
     d = np.zeros((n, N_OUT))
     d[:, :30] = 2.0 * w * r / w.sum() / n
@@ -207,14 +213,19 @@ def node_frame(mass, r_vir, v_vir, f_baryon, unit_energy):
 
 
 def _baseline(y0, rate, dt, sc):
-    """Asinh-space increment of the explicit Euler step y0 + rate*dt."""
+    """First-order increment in asinh space: the Euler step taken in y for a
+    growing quantity, taken in u = asinh(y/s) for a decaying one."""
     du = np.zeros(30)
     for v in range(30):
         role = VAR_ROLES[v]
         if role == ROLE_ENERGY:
             du[v] = math.asinh(rate[v] * dt / sc[role])
         elif role not in (ROLE_CONST, ROLE_SPIN):
-            du[v] = math.asinh((y0[v] + rate[v] * dt) / sc[role]) - math.asinh(y0[v] / sc[role])
+            s = sc[role]
+            if rate[v] >= 0.0:
+                du[v] = math.asinh((y0[v] + rate[v] * dt) / s) - math.asinh(y0[v] / s)
+            else:
+                du[v] = rate[v] * dt / math.sqrt(s * s + y0[v] * y0[v])
     return du
 
 
@@ -252,6 +263,14 @@ def encode_raw(y0, y1, t0, t1, halo, node_par, f_baryon, unit_energy, v_suppress
     return x, t
 
 
+def _sinh(x):
+    """sinh with IEEE overflow to +-inf (as the device function)."""
+    try:
+        return math.sinh(x)
+    except OverflowError:
+        return math.copysign(math.inf, x)
+
+
 def decode(y0, d, dt, halo, node_par, f_baryon, unit_energy, rate=None):
     """Inverse of the target map: state at t1 from y0 and raw residuals d."""
     sc = node_frame(halo["mass"], halo["r_vir"], halo["v_vir"], f_baryon, unit_energy)
@@ -264,7 +283,7 @@ def decode(y0, d, dt, halo, node_par, f_baryon, unit_energy, rate=None):
         if role == ROLE_SPIN:
             y1[v] = y0[v] + (0.0 if node_par[4] != 0.0 else node_par[3]) * dt
         elif role == ROLE_ENERGY:
-            y1[v] = y0[v] + sc[role] * math.sinh(du[v] + d[v])
+            y1[v] = y0[v] + sc[role] * _sinh(du[v] + d[v])
         else:
-            y1[v] = sc[role] * math.sinh(math.asinh(y0[v] / sc[role]) + du[v] + d[v])
+            y1[v] = sc[role] * _sinh(math.asinh(y0[v] / sc[role]) + du[v] + d[v])
     return y1
