@@ -29,6 +29,16 @@ sys.path.insert(0, ROOT)
 METRIC = "node-Gyr evolved/sec: FP64 batched RK vs MLP+fallback at ≤1% M* error"
 FP64_PEAK_TFLOPS = 78.6
 
+# HBM bytes per evolve launch on the 1e5-node isolated-halo batch, by launch
+# shape, from rocprofv3 --pmc FETCH_SIZE and --pmc WRITE_SIZE runs of this
+# script (profiles/evolve_kernel_hbm_traffic_isolated_halos_1e5.md), each
+# counter scaled by what it reads for one elementwise kernel that reads and
+# writes 1 GiB (FETCH_SIZE x 2.0000, WRITE_SIZE x 1.0000).
+HBM_TRAFFIC_1E5 = {
+    (64, 1): (228573.5 * 2.0 + 212418.6 * 1.0) * 1024,
+    (64, 2): (323554.8 * 2.0 + 689342.9 * 1.0) * 1024,
+}
+
 
 def log(msg):
     print(msg, file=sys.stderr, flush=True)
@@ -249,12 +259,14 @@ def trees_leg(physics, table, tol_abs, tol_rel, args):
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
-    ap.add_argument("--steps", type=int, default=5)
-    ap.add_argument("--warmup", type=int, default=2)
+    ap.add_argument("--steps", type=int, default=20)
+    ap.add_argument("--warmup", type=int, default=4)
     ap.add_argument("--nodes", type=int, default=100000)
     ap.add_argument("--seed", type=int, default=0)
-    ap.add_argument("--streams", type=int, default=2,
+    ap.add_argument("--streams", type=int, default=4,
                     help="independent batches alternate over this many streams and evolver handles")
+    ap.add_argument("--launch", default="auto",
+                    help="launch shape 'nodes_per_workgroup,workgroups_per_cu', or 'auto' for the autotuner")
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
     ap.add_argument("--cpu-seconds", type=float, default=15.0)
     ap.add_argument("--no-surrogate", action="store_true", help="skip the surrogate training/inference legs")
@@ -301,7 +313,11 @@ def main():
         torch.cuda.synchronize()
         log(f"copy calibration: {n_el * 8} bytes read and written by one elementwise kernel")
         del src, dst
-    launch = ev.autotune(y0, npar, t0, t1, h0)
+    if args.launch == "auto":
+        launch = ev.autotune(y0, npar, t0, t1, h0)
+    else:
+        launch = tuple(int(v) for v in args.launch.split(","))
+        ev.set_launch(*launch)
     for e in evs[1:]:
         e.set_launch(*launch)
     log(f"autotuned launch: {launch[0]} nodes per workgroup, {launch[1]} workgroups per CU")
@@ -350,6 +366,7 @@ def main():
     cnt = counters[0].view(3, n).cpu().numpy()
     attempts = int(cnt[0].sum())
     flop_step = gflops.attempt_flops()
+    state_bytes = 8 * (gp.NVAR + npar.shape[0] + 3) + 8 * (2 * gp.NVAR + 1) + 4 + 3 * 8
     achieved_tflops = attempts * flop_step / (kernel_ms * 1e-3) / 1e12
     aggregate_tflops = attempts * args.steps * flop_step / wall / 1e12
     log(f"step {ms_per_step:.3f} ms (one launch {kernel_ms:.3f} ms, {n_streams} streams), attempts {attempts}, "
@@ -382,11 +399,17 @@ def main():
             "peak": FP64_PEAK_TFLOPS,
             "unit": "TFLOP/s",
             "frac": achieved_tflops / FP64_PEAK_TFLOPS,
-            "traffic": None,
+            "traffic": HBM_TRAFFIC_1E5.get(tuple(launch)) if n == 100000 and args.seed == 0 else None,
+            "traffic_unit": "bytes per launch (rocprofv3 counters, calibrated)",
             "flop_per_attempt": flop_step,
             "attempts_per_launch": attempts,
             "kernel_ms": kernel_ms,
             "aggregate_tflops": aggregate_tflops,
+            # node state in and out of HBM: y, node parameters, t0, t1, h in;
+            # y, h, scales, status, counters out
+            "state_bytes_per_node": state_bytes,
+            "state_gbs": state_bytes * n / (kernel_ms * 1e-3) / 1e9,
+            "hbm_peak_gbs": 8000.0,
         },
     }
     del ys, hs

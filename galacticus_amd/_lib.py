@@ -42,6 +42,11 @@ def load():
         return _lib
     if not os.path.exists(LIB_PATH):
         raise GalError(f"{LIB_PATH} is missing: build the HIP library first (python -m galacticus_amd.build)")
+    # One HIP runtime per process: torch (device memory and streams) ships its
+    # own, and a library that binds the system runtime first leaves torch
+    # without a device.  Import torch before the library resolves its HIP
+    # dependency.
+    import torch  # noqa: F401
     L = ctypes.CDLL(LIB_PATH)
     vp, dp, ip, i64p = ctypes.c_void_p, ctypes.POINTER(ctypes.c_double), ctypes.POINTER(ctypes.c_int), ctypes.c_void_p
     c_int, c_double = ctypes.c_int, ctypes.c_double

@@ -25,6 +25,8 @@ struct gal_ode_batch {
   unsigned int* d_work = nullptr;
   double* d_ywork = nullptr;  // lane-private state/scales for two workgroups per CU
   int ywork_lanes = 0;
+  double* d_kwork = nullptr;  // lane-private stage derivatives for four workgroups per CU
+  int kwork_lanes = 0;
   int* d_order = nullptr;
   int* d_index = nullptr;
   float* d_key = nullptr;

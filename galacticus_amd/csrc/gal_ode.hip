@@ -27,10 +27,12 @@
 // Memory.  State is structure-of-arrays in HBM (y[v*ld + i]); it is read once
 // when a node is claimed and written once when it finishes.  Within a node,
 // the RHS input and output stay in registers, and the five stage
-// derivatives k1..k5, the state y and the error scales live in LDS in a
+// derivatives k1..k5, the state y and the error scales live in a
 // lane-private, bank-conflict-free layout L[(vec*30 + v)*BLOCK + lane]
-// (7*30*8 = 1680 bytes per lane, 105 KiB per 64-lane workgroup).  Nothing
-// is addressable private memory, so the long dependent chain of a node never
+// (7*30*8 = 1680 bytes per lane): all in LDS (105 KiB per 64-lane
+// workgroup), or part of them in an HBM workspace of the same layout so that
+// two or four workgroups share a CU (see evolve_kernel).  Nothing is
+// addressable private memory, so the long dependent chain of a node never
 // waits on scratch.
 #include <hip/hip_runtime.h>
 #include <cstdint>
@@ -69,6 +71,7 @@ struct EvolveArgs {
   unsigned int* work;
   double* ywork;  // per-lane state and scales in HBM (variant without them in LDS)
   int ystride;    // lanes in the grid (ywork holds 2*30 doubles per lane)
+  double* kwork;  // per-lane stage derivatives in HBM (variant without them in LDS)
   const double* P;
   const double* tab;
   DerivedConstants D;
@@ -104,26 +107,56 @@ __device__ __forceinline__ void write_counters(const EvolveArgs& a, int node, lo
   }
 }
 
-// YLDS: the state and the error scales of each lane live in LDS beside the
-// five stage derivatives (105 KiB per 64 lanes: one workgroup per CU).
-// Otherwise they live in a lane-private HBM workspace laid out like the LDS
-// (coalesced, cache-resident) and a workgroup needs 75 KiB of LDS, so two
-// workgroups share a CU and use two of its four SIMDs.
-template <int BLOCK, bool YLDS>
+// Three placements of the lane-private vectors (each [vector][variable][lane]
+// per workgroup, so a wave touches 64 consecutive doubles per variable):
+//  YLDS && KLDS:  stage derivatives, state and error scales in LDS (105 KiB
+//                 per 64 lanes: one workgroup per CU);
+//  !YLDS && KLDS: state and scales in an HBM workspace, stages in LDS
+//                 (75 KiB: two workgroups per CU);
+//  YLDS && !KLDS: stages in an HBM workspace, state and scales in LDS
+//                 (30 KiB: four workgroups per CU, one wave on every SIMD).
+// The arithmetic is the same in all three; only where the operands wait
+// between phases differs.
+template <int BLOCK, bool YLDS, bool KLDS>
 __global__ __launch_bounds__(BLOCK, 1) void evolve_kernel(EvolveArgs a) {
   extern __shared__ double lds_k[];
-  double* __restrict__ K = lds_k + threadIdx.x;   // K[(s*kNV + v)*BLOCK]
+  double* __restrict__ K = lds_k + threadIdx.x;   // K[(s*kNV + v)*BLOCK] when KLDS
+  // In the HBM workspace the stage vectors are reached through a buffer
+  // resource over the workgroup's slab: the lane offset sits in one VGPR and
+  // the per-vector offsets in SGPRs, so 150 addresses cost no registers.
+  const __amdgpu_buffer_rsrc_t krs = __builtin_amdgcn_make_buffer_rsrc(
+      KLDS ? nullptr : a.kwork + (size_t)blockIdx.x * (kStages * kNV * BLOCK), (short)0,
+      (int)(sizeof(double) * kStages * kNV * BLOCK), 0x00020000);
+  const int k_lane = (int)(threadIdx.x * sizeof(double));
+  using B64 = decltype(__builtin_amdgcn_raw_buffer_load_b64(krs, 0, 0, 0));
+  auto kget = [&](int st, int v) -> double {
+    if constexpr (KLDS) {
+      return K[(st * kNV + v) * BLOCK];
+    } else {
+      return __builtin_bit_cast(double, __builtin_amdgcn_raw_buffer_load_b64(
+                                            krs, k_lane, (int)sizeof(double) * (st * kNV + v) * BLOCK, 0));
+    }
+  };
+  auto kset = [&](int st, int v, double x) {
+    if constexpr (KLDS) {
+      K[(st * kNV + v) * BLOCK] = x;
+    } else {
+      const int su = __builtin_amdgcn_readfirstlane(st);   // the phase is uniform over the wave
+      __builtin_amdgcn_raw_buffer_store_b64(__builtin_bit_cast(B64, x), krs, k_lane,
+                                            (int)sizeof(double) * (su * kNV + v) * BLOCK, 0);
+    }
+  };
   const double* __restrict__ P = a.P;
   const int mode = a.mode;
   const bool reuse = (a.flags & GAL_FLAG_REUSE_STEP_SIZE) != 0;
   const bool enforce = (a.flags & GAL_FLAG_ENFORCE_NON_NEGATIVITY) != 0;
   const bool scales_given = (a.flags & GAL_FLAG_SCALES_PROVIDED) != 0;
-#define KS(s, v) K[((s) * kNV + (v)) * BLOCK]
+#define KS(s, v) kget((s), (v))
 
   // Both layouts are [vector][variable][lane] per workgroup: a wave reads
   // 64 consecutive doubles per variable.
   constexpr int ys = BLOCK;
-  double* __restrict__ YL = YLDS ? lds_k + kStages * kNV * BLOCK + threadIdx.x
+  double* __restrict__ YL = YLDS ? lds_k + (KLDS ? kStages * kNV * BLOCK : 0) + threadIdx.x
                                  : a.ywork + (size_t)blockIdx.x * (2 * kNV * BLOCK) + threadIdx.x;
   double* __restrict__ SL = YL + kNV * BLOCK;
 #define Y(v) YL[(v) * ys]
@@ -256,7 +289,7 @@ __global__ __launch_bounds__(BLOCK, 1) void evolve_kernel(EvolveArgs a) {
     if (phase < 4) {
       if (active) {
 #pragma unroll
-        for (int v = 0; v < kNV; ++v) KS(phase + 1, v) = kout[v];
+        for (int v = 0; v < kNV; ++v) kset(phase + 1, v, kout[v]);
       }
       ++phase;
       continue;
@@ -264,7 +297,7 @@ __global__ __launch_bounds__(BLOCK, 1) void evolve_kernel(EvolveArgs a) {
     if (phase == 5) {
       if (eval_pending) {
 #pragma unroll
-        for (int v = 0; v < kNV; ++v) KS(0, v) = kout[v];
+        for (int v = 0; v < kNV; ++v) kset(0, v, kout[v]);
         eval_pending = false;
         if (post_pending) {
           post_step(P, a.D, np, mode, t, &Y(0), ys);   // GSL_CONTINUE: no evolver reset
@@ -438,9 +471,9 @@ __global__ void cost_probe_kernel(int n_items, const int* __restrict__ subset, i
   index[j] = i;
 }
 
-template <int BLOCK, bool YLDS>
+template <int BLOCK, bool YLDS, bool KLDS>
 constexpr size_t lds_bytes() {
-  return sizeof(double) * (YLDS ? kLdsVectors : kStages) * kNV * BLOCK;
+  return sizeof(double) * ((YLDS ? 2 : 0) + (KLDS ? kStages : 0)) * kNV * BLOCK;
 }
 
 }  // namespace
@@ -558,18 +591,23 @@ int gal_ode_batch_create(int n_max, double tol_abs, double tol_rel, int flags, g
     return GAL_E_NOMEM;
   }
   h->ywork_lanes = ncu * 2 * 64;
-  if (hipMalloc(&h->d_ywork, sizeof(double) * 2 * kNV * (size_t)h->ywork_lanes) != hipSuccess) {
+  h->kwork_lanes = ncu * 4 * 64;
+  if (hipMalloc(&h->d_ywork, sizeof(double) * 2 * kNV * (size_t)h->ywork_lanes) != hipSuccess ||
+      hipMalloc(&h->d_kwork, sizeof(double) * kStages * kNV * (size_t)h->kwork_lanes) != hipSuccess) {
     gal_ode_batch_destroy(h);
     return GAL_E_NOMEM;
   }
   static bool attrs_set = false;
   if (!attrs_set) {
-    static_assert(lds_bytes<64, true>() <= 163840, "lane-private LDS vectors must fit 160 KiB");
-    static_assert(2 * lds_bytes<64, false>() <= 163840, "two workgroups per CU must fit 160 KiB");
-    if (hipFuncSetAttribute((const void*)evolve_kernel<64, true>, hipFuncAttributeMaxDynamicSharedMemorySize,
-                            (int)lds_bytes<64, true>()) != hipSuccess ||
-        hipFuncSetAttribute((const void*)evolve_kernel<64, false>, hipFuncAttributeMaxDynamicSharedMemorySize,
-                            (int)lds_bytes<64, false>()) != hipSuccess) {
+    static_assert(lds_bytes<64, true, true>() <= 163840, "lane-private LDS vectors must fit 160 KiB");
+    static_assert(2 * lds_bytes<64, false, true>() <= 163840, "two workgroups per CU must fit 160 KiB");
+    static_assert(4 * lds_bytes<64, true, false>() <= 163840, "four workgroups per CU must fit 160 KiB");
+    if (hipFuncSetAttribute((const void*)evolve_kernel<64, true, true>, hipFuncAttributeMaxDynamicSharedMemorySize,
+                            (int)lds_bytes<64, true, true>()) != hipSuccess ||
+        hipFuncSetAttribute((const void*)evolve_kernel<64, false, true>, hipFuncAttributeMaxDynamicSharedMemorySize,
+                            (int)lds_bytes<64, false, true>()) != hipSuccess ||
+        hipFuncSetAttribute((const void*)evolve_kernel<64, true, false>, hipFuncAttributeMaxDynamicSharedMemorySize,
+                            (int)lds_bytes<64, true, false>()) != hipSuccess) {
       (void)hipGetLastError();
       gal_ode_batch_destroy(h);
       return GAL_E_HIP;
@@ -594,6 +632,7 @@ int gal_ode_batch_destroy(gal_ode_batch* h) {
   hipFree(h->d_table);
   hipFree(h->d_work);
   hipFree(h->d_ywork);
+  hipFree(h->d_kwork);
   hipFree(h->d_order);
   hipFree(h->d_index);
   hipFree(h->d_key);
@@ -646,9 +685,10 @@ int gal_ode_batch_set_cooling_table(gal_ode_batch* h, const double* ln_lambda, i
 int gal_ode_batch_set_launch(gal_ode_batch* h, int nodes_per_block, int blocks_per_cu) {
   if (!h) return GAL_E_INVALID;
   if (nodes_per_block != 64) return GAL_E_INVALID;
-  // one workgroup per CU keeps state and scales in LDS; two keep them in the
-  // HBM workspace; the stage derivatives of a CU must fit its 160 KiB of LDS
-  if (blocks_per_cu < 1 || blocks_per_cu > 2) return GAL_E_INVALID;
+  // one workgroup per CU keeps every lane vector in LDS; two keep state and
+  // scales in the HBM workspace; three or four keep the stage derivatives
+  // there (the LDS of a CU is 160 KiB)
+  if (blocks_per_cu < 1 || blocks_per_cu > 4) return GAL_E_INVALID;
   h->nodes_per_block = nodes_per_block;
   h->blocks_per_cu = blocks_per_cu;
   return GAL_OK;
@@ -657,15 +697,21 @@ int gal_ode_batch_set_launch(gal_ode_batch* h, int nodes_per_block, int blocks_p
 static int launch_evolve(gal_ode_batch* h, const EvolveArgs& args, hipStream_t s) {
   const int blocks = std::max(1, h->n_cus * h->blocks_per_cu);
   hipLaunchKernelGGL(reset_counter_kernel, dim3(1), dim3(64), 0, s, args.work);
-  constexpr size_t lds_y = lds_bytes<64, true>(), lds_k = lds_bytes<64, false>();
+  constexpr size_t lds_all = lds_bytes<64, true, true>(), lds_k = lds_bytes<64, false, true>(),
+                   lds_y = lds_bytes<64, true, false>();
   if (h->blocks_per_cu == 1) {
-    hipLaunchKernelGGL((evolve_kernel<64, true>), dim3(blocks), dim3(64), lds_y, s, args);
-  } else {
+    hipLaunchKernelGGL((evolve_kernel<64, true, true>), dim3(blocks), dim3(64), lds_all, s, args);
+  } else if (h->blocks_per_cu == 2) {
     EvolveArgs b = args;
     b.ywork = h->d_ywork;
     b.ystride = blocks * 64;
     if (b.ystride > h->ywork_lanes) return GAL_E_INVALID;
-    hipLaunchKernelGGL((evolve_kernel<64, false>), dim3(blocks), dim3(64), lds_k, s, b);
+    hipLaunchKernelGGL((evolve_kernel<64, false, true>), dim3(blocks), dim3(64), lds_k, s, b);
+  } else {
+    EvolveArgs b = args;
+    b.kwork = h->d_kwork;
+    if (blocks * 64 > h->kwork_lanes) return GAL_E_INVALID;
+    hipLaunchKernelGGL((evolve_kernel<64, true, false>), dim3(blocks), dim3(64), lds_y, s, b);
   }
   HIP_TRY(hipGetLastError());
   return GAL_OK;
@@ -693,6 +739,7 @@ static int make_args(gal_ode_batch* h, int n, int ld, double* y, const double* n
   a.work = h->d_work;
   a.ywork = nullptr;
   a.ystride = 0;
+  a.kwork = nullptr;
   a.P = h->d_params;
   a.tab = h->d_table;
   a.D = h->D;
@@ -793,7 +840,7 @@ int gal_ode_batch_autotune(gal_ode_batch* h, int n, int ld, double* y, const dou
     hipFree(wy); hipFree(wh); hipFree(wsc); hipFree(wst);
     return GAL_E_NOMEM;
   }
-  const int shapes[][2] = {{64, 1}, {64, 2}};
+  const int shapes[][2] = {{64, 1}, {64, 2}, {64, 4}};
   float best = 1e30f;
   int bn = h->nodes_per_block, bb = h->blocks_per_cu;
   hipEvent_t e0, e1;

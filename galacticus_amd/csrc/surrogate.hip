@@ -411,8 +411,12 @@ __device__ __forceinline__ void euler_baseline(const double* __restrict__ P, con
       d = asinh(rate[v] * dt / f.s_energy);
     } else if (role != R_CONST && role != R_SPIN) {
       const double s = var_scale(f, role);
-      d = rate[v] >= 0.0 ? asinh((y0[v] + rate[v] * dt) / s) - asinh(y0[v] / s)
-                         : rate[v] * dt * rsqrt(s * s + y0[v] * y0[v]);
+      if (rate[v] >= 0.0)
+        d = asinh((y0[v] + rate[v] * dt) / s) - asinh(y0[v] / s);          // linear growth
+      else if (y0[v] > 0.0)
+        d = asinh(y0[v] * exp(rate[v] * dt / y0[v]) / s) - asinh(y0[v] / s);  // decay on y/|f|
+      else
+        d = rate[v] * dt * rsqrt(s * s + y0[v] * y0[v]);
     }
     du[v] = d;
   }

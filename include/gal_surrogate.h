@@ -28,8 +28,11 @@
  *   Targets 0..29: u_v(t1) - u_v(t0) with the t0 scales (the radiated
  *   energy uses asinh(dE / s_E)), minus a first-order increment from one
  *   RHS evaluation f = f(t0, y0): asinh((y + f dt)/s) - asinh(y/s) where
- *   f >= 0, f dt / sqrt(s^2 + y^2) where f < 0, asinh(f dt / s_E) for the
- *   energy: the network learns the residual of a first-order step.  Column 30 is the error head; a nonzero column 31
+ *   f >= 0 (linear growth), asinh(y exp(f dt / y)/s) - asinh(y/s) where
+ *   f < 0 < y (exponential decay on the time scale y/|f|), f dt /
+ *   sqrt(s^2 + y^2) where f < 0 and y <= 0, asinh(f dt / s_E) for the
+ *   energy: the network learns the residual of a first-order step.
+ *   Column 30 is the error head; a nonzero column 31
  *   marks a training sample whose targets were clipped (its error counts
  *   as at least 1 in the head's loss).
  *   Variables with zero rate are carried over, the spin angular momentum is

@@ -214,7 +214,8 @@ def node_frame(mass, r_vir, v_vir, f_baryon, unit_energy):
 
 def _baseline(y0, rate, dt, sc):
     """First-order increment in asinh space: the Euler step taken in y for a
-    growing quantity, taken in u = asinh(y/s) for a decaying one."""
+    growing quantity, exponential decay on the time scale y/|f| for a
+    decaying positive one, the Euler step in u = asinh(y/s) otherwise."""
     du = np.zeros(30)
     for v in range(30):
         role = VAR_ROLES[v]
@@ -224,6 +225,8 @@ def _baseline(y0, rate, dt, sc):
             s = sc[role]
             if rate[v] >= 0.0:
                 du[v] = math.asinh((y0[v] + rate[v] * dt) / s) - math.asinh(y0[v] / s)
+            elif y0[v] > 0.0:
+                du[v] = math.asinh(y0[v] * math.exp(rate[v] * dt / y0[v]) / s) - math.asinh(y0[v] / s)
             else:
                 du[v] = rate[v] * dt / math.sqrt(s * s + y0[v] * y0[v])
     return du

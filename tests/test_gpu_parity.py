@@ -193,20 +193,22 @@ def test_evolve_matches_oracle(torch_cuda, cooling_table, reuse):
 
 
 def test_launch_variants_bit_identical(torch_cuda, cooling_table):
-    """One workgroup per CU (state in LDS) and two per CU (state in the HBM
-    workspace) run the same arithmetic: identical results, bit for bit."""
+    """One workgroup per CU (every lane vector in LDS), two (state in the HBM
+    workspace) and four (stage derivatives in the HBM workspace) run the same
+    arithmetic: identical results, bit for bit."""
     torch = torch_cuda
     physics = gp.physics_params()
     b = wl.isolated_halos(4000, seed=21, params=physics)
     ev = _evolver(b.n, physics)
     d = lambda a: torch.from_numpy(np.ascontiguousarray(a)).cuda()
     out = []
-    for shape in ((64, 1), (64, 2)):
+    for shape in ((64, 1), (64, 2), (64, 4)):
         ev.set_launch(*shape)
         y, h = d(b.y), d(b.h_step)
         st = ev.evolve_device(y, d(b.node_par), d(b.t_start), d(b.t_end), h)
         torch.cuda.synchronize()
         out.append((y.cpu(), h.cpu(), st.cpu()))
-    assert torch.equal(out[0][0], out[1][0]) and torch.equal(out[0][1], out[1][1]) and torch.equal(out[0][2], out[1][2])
+    for o in out[1:]:
+        assert torch.equal(out[0][0], o[0]) and torch.equal(out[0][1], o[1]) and torch.equal(out[0][2], o[2])
     shape = ev.autotune(d(b.y), d(b.node_par), d(b.t_start), d(b.t_end), d(b.h_step))
-    assert shape in ((64, 1), (64, 2))
+    assert shape in ((64, 1), (64, 2), (64, 4))

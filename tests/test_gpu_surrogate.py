@@ -250,13 +250,23 @@ def test_accepted_nodes_get_the_decoded_network_output(torch_cuda, cooling_table
     torch = torch_cuda
     from galacticus_amd.surrogate import SurrogateEvolver, GateConfig, PairSet, encode
     p, b, t1, ev, mlp, norm, d = _surrogate_setup(torch, cooling_table, n=600, seed=8)
+    # An untrained network's outputs become a small positive offset on top of
+    # the first-order step (every decoded state then stays non-negative); the
+    # head threshold at the median splits the nodes into accepted and rejected.
+    norm = norm.clone()
+    norm[2, :30] = 0.01
+    norm[3, :30] = 1.0e-3
     pairs = PairSet(d(b.y), None, d(b.node_par), d(b.t_start), d(t1))
     x, _, _ = encode(ev, pairs, norm=norm, with_target=False)
     out = mlp.forward(x).cpu().numpy().astype(np.float64)
     nm = norm.cpu().numpy().astype(np.float64)
+    hs = np.unique(out[:, 30])
+    k = np.searchsorted(hs, np.median(out[:, 30]))
+    thr = 0.5 * (hs[max(k - 1, 0)] + hs[k])
     y_rk = d(b.y)
     ev.evolve_device(y_rk, d(b.node_par), d(b.t_start), d(t1), d(b.h_step))
-    se = SurrogateEvolver(ev, mlp, norm, GateConfig(error_max=1e300, mass_tolerance=1e300, metal_tolerance=1e300))
+    se = SurrogateEvolver(ev, mlp, norm, GateConfig(error_max=math.exp(thr), mass_tolerance=1e300,
+                                                    metal_tolerance=1e300))
     y, h = d(b.y), d(b.h_step)
     st, acc, n_rej = se.evolve_device(y, d(b.node_par), d(b.t_start), d(t1), h)
     yh, acc, y_rk = y.cpu().numpy(), acc.cpu().numpy(), y_rk.cpu().numpy()
@@ -269,7 +279,8 @@ def test_accepted_nodes_get_the_decoded_network_output(torch_cuda, cooling_table
         halo = _halo(p, cooling_table, b.node_par[:, i], b.t_start[i], b.y[:, i])
         dd = nm[2, :30] + nm[3, :30] * out[i, :30]
         y1 = om.decode(b.y[:, i], dd, t1[i] - b.t_start[i], halo, b.node_par[:, i], fb, UNIT_ENERGY, rate[:, i])
-        ok = bool(np.isfinite(y1).all() and (y1[nonneg] >= 0).all())
+        ok = bool(np.isfinite(y1).all() and (y1[nonneg] >= 0).all() and out[i, 30] <= thr)
+        assert bool(acc[i]) == ok, i
         if acc[i]:
             n_acc += 1
             assert ok, i
