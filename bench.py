@@ -149,8 +149,12 @@ def surrogate_leg(physics, table, tol_abs, tol_rel, args):
     # the same surrogate on chained node histories from z=5 to z=0
     chain = chained_histories(ev, se, physics, args)
     rk_rate, sur_rate = gyr / t_rk, gyr / t_sur
+    piped = pipelined_surrogate(ev, mlp, norm, gate, test, h0, launch, physics, table, tol_abs, tol_rel, args)
     log(f"surrogate: RK {rk_rate:.4g} node-Gyr/s, surrogate+fallback {sur_rate:.4g} ({sur_rate / rk_rate:.2f}x), "
         f"rejected {n_rej / n:.3f}, median M* error {float(ms_err.median()):.2e}")
+    log(f"surrogate pipelined over {piped['streams']} streams: RK {piped['rk_s_per_batch'] * 1e3:.2f} ms/batch, "
+        f"surrogate+fallback {piped['surrogate_s_per_batch'] * 1e3:.2f} ms/batch "
+        f"({piped['rk_s_per_batch'] / piped['surrogate_s_per_batch']:.2f}x)")
     return {
         "workload": "node-interval steps from node histories (simple timestep shortened by U(0.25,1)), "
                     "FP64 integrator vs bf16 MLP + gate + FP64 fallback",
@@ -166,12 +170,72 @@ def surrogate_leg(physics, table, tol_abs, tol_rel, args):
                       "median_stellar_mass_error_all": float(ms_err.median()),
                       "accepted_relative_error": per_q},
         "chained_to_z0": chain,
+        "pipelined": dict(piped, rk_node_gyr_per_s=gyr / piped["rk_s_per_batch"],
+                          surrogate_node_gyr_per_s=gyr / piped["surrogate_s_per_batch"],
+                          speedup=piped["rk_s_per_batch"] / piped["surrogate_s_per_batch"]),
     }
 
 
 def dataclasses_asdict(x):
     import dataclasses
     return dataclasses.asdict(x)
+
+
+def pipelined_surrogate(ev, mlp, norm, gate, test, h0, launch, physics, table, tol_abs, tol_rel, args):
+    """Throughput of both paths as bench's main leg measures the integrator:
+    independent copies of the evaluation batch alternate over streams, each
+    stream with its own integrator handle (and network handle), so one
+    batch's tail of stiff nodes overlaps the next batch's start."""
+    import torch
+    from galacticus_amd import surrogate as sg
+    from galacticus_amd.evolver import NodeEvolverBatched
+    n_streams, batches = max(1, args.streams), args.pipeline_batches
+    n = test.n
+    evs = [ev] + [NodeEvolverBatched(n, physics, table, odeToleranceAbsolute=tol_abs, odeToleranceRelative=tol_rel)
+                  for _ in range(n_streams - 1)]
+    for e in evs[1:]:
+        e.set_launch(*launch)
+    params = mlp.get_params()
+    mlps = [mlp] + [sg.SurrogateMLP(8192, params=params) for _ in range(n_streams - 1)]
+    ses = [sg.SurrogateEvolver(e, m, norm, gate) for e, m in zip(evs, mlps)]
+    streams = [torch.cuda.Stream() for _ in range(n_streams)]
+    status = [torch.empty(n, dtype=torch.int32, device=test.t0.device) for _ in range(n_streams)]
+    accepted = [torch.empty(n, dtype=torch.int32, device=test.t0.device) for _ in range(n_streams)]
+    out = {}
+    for name in ("rk", "surrogate"):
+        ys = [test.y0.clone() for _ in range(batches)]
+        hs = [h0.clone() for _ in range(batches)]
+        n_rej = 0
+
+        def launch_one(k):
+            j = k % n_streams
+            if name == "rk":
+                evs[j].evolve_device(ys[k], test.node_par, test.t0, test.t1, hs[k], status=status[j], stream=streams[j])
+            else:
+                ses[j].evolve_device(ys[k], test.node_par, test.t0, test.t1, hs[k], status=status[j],
+                                     accepted=accepted[j], stream=streams[j], count_rejected=False)
+        launch_one(0)                      # warm-up (allocations of first use)
+        torch.cuda.synchronize()
+        ys[0] = test.y0.clone()
+        hs[0] = h0.clone()
+        torch.cuda.synchronize()
+        t = time.perf_counter()
+        for k in range(batches):
+            launch_one(k)
+        torch.cuda.synchronize()
+        wall = time.perf_counter() - t
+        if name == "surrogate":
+            n_rej = int((accepted[0] == 0).sum())
+        out[f"{name}_s_per_batch"] = wall / batches
+        if name == "surrogate":
+            out["rejected_fraction_last_batch"] = n_rej / n
+        ok = all(bool((st == 0).all()) for st in status)
+        out[f"{name}_statuses_ok"] = ok
+        del ys, hs
+    out.update({"streams": n_streams, "batches": batches, "intervals_per_batch": n})
+    for m in mlps[1:]:
+        m.close()
+    return out
 
 
 def chained_histories(ev, se, physics, args, n=20000):
@@ -274,6 +338,8 @@ def main():
     ap.add_argument("--train-pairs", type=int, default=4000000)
     ap.add_argument("--train-epochs", type=int, default=20)
     ap.add_argument("--eval-pairs", type=int, default=1000000)
+    ap.add_argument("--pipeline-batches", type=int, default=8,
+                    help="evaluation batches per path in the pipelined surrogate comparison")
     ap.add_argument("--copy-calibration-mib", type=int, default=0,
                     help="run one elementwise copy of this many MiB first (reference kernel for HBM counters)")
     args = ap.parse_args()

@@ -80,7 +80,8 @@ SURROGATE_EXPORTS = [
 class Gate(ctypes.Structure):
     """gal_surrogate_gate of include/gal_surrogate.h."""
     _fields_ = [("log_error_max", ctypes.c_double), ("mass_tolerance", ctypes.c_double),
-                ("metal_tolerance", ctypes.c_double), ("bh_slack", ctypes.c_double)]
+                ("metal_tolerance", ctypes.c_double), ("bh_slack", ctypes.c_double),
+                ("negative_tolerance", ctypes.c_double)]
 
 
 def _load_surrogate(L):

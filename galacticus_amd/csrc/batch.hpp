@@ -56,4 +56,5 @@ struct gal_ode_batch {
 // has GAL_FLAG_COST_ORDER.
 int gal_ode_evolve_subset(gal_ode_batch* h, int n, int ld, double* y, const double* node_par, const double* t_start,
                           const double* t_end, double* h_step, double* scale, int* status, int64_t* counters,
-                          const int* subset, int n_subset, hipStream_t s);
+                          const int* subset, int n_subset, hipStream_t s,
+                          const int* n_subset_dev = nullptr);

@@ -59,6 +59,7 @@ constexpr int kLdsVectors = kStages + 2;  // k1..k5, y, scales
 struct EvolveArgs {
   int n, ld;
   int n_claim;  // claims are limited to n_claim entries of order[] (n when order is null)
+  const int* n_claim_dev;  // if set, the limit is read from device memory instead
   double* y;
   const double* np;
   const double* t_start;
@@ -168,6 +169,7 @@ __global__ __launch_bounds__(BLOCK, 1) void evolve_kernel(EvolveArgs a) {
   double t = 0.0, t0 = 0.0, t1 = 0.0, h = 0.0, h0 = 0.0, h_node = 0.0, sign = 1.0;
   int trial = 0;
   long long n_attempt = 0, n_accept = 0, n_rhs = 0;
+  const int n_claim = a.n_claim_dev ? *a.n_claim_dev : a.n_claim;
 
   int phase = 5;
   for (;;) {
@@ -176,7 +178,7 @@ __global__ __launch_bounds__(BLOCK, 1) void evolve_kernel(EvolveArgs a) {
     if (phase == 5) {
       // ---- claim nodes for lanes that need one (convergent point) ------
       while (__any(want)) {
-        const int idx = wave_claim(want, a.work, a.n_claim, a.order);
+        const int idx = wave_claim(want, a.work, n_claim, a.order);
         if (want) {
           if (idx < 0) {
             want = false;
@@ -439,10 +441,16 @@ __global__ void cost_probe_kernel(int n_items, const int* __restrict__ subset, i
                                   const double* __restrict__ npa, const double* __restrict__ t_start,
                                   const double* __restrict__ t_end, const double* __restrict__ scale_in,
                                   const double* __restrict__ P, const double* __restrict__ tab, DerivedConstants D,
-                                  CoolingDesc cd, int mode, int scales_given, float* __restrict__ key,
-                                  int* __restrict__ index) {
+                                  CoolingDesc cd, int mode, int scales_given, const int* __restrict__ n_valid,
+                                  float* __restrict__ key, int* __restrict__ index) {
   const int j = blockIdx.x * blockDim.x + threadIdx.x;
   if (j >= n_items) return;
+  if (n_valid && j >= *n_valid) {
+    // beyond the count the caller's list is unset: sorted last, never claimed
+    key[j] = -1.0f;
+    index[j] = 0;
+    return;
+  }
   const int i = subset ? subset[j] : j;
   double yy[kNV], f[kNV], sc[kNV];
 #pragma unroll
@@ -736,6 +744,7 @@ static int make_args(gal_ode_batch* h, int n, int ld, double* y, const double* n
   a.status = status;
   a.counters = reinterpret_cast<long long*>(counters);
   a.order = nullptr;
+  a.n_claim_dev = nullptr;
   a.work = h->d_work;
   a.ywork = nullptr;
   a.ystride = 0;
@@ -881,20 +890,21 @@ int gal_ode_batch_autotune(gal_ode_batch* h, int n, int ld, double* y, const dou
 
 int gal_ode_evolve_subset(gal_ode_batch* h, int n, int ld, double* y, const double* node_par, const double* t_start,
                           const double* t_end, double* h_step, double* scale, int* status, int64_t* counters,
-                          const int* subset, int n_subset, hipStream_t s) {
+                          const int* subset, int n_subset, hipStream_t s, const int* n_subset_dev) {
   EvolveArgs a;
   int rc = make_args(h, n, ld, y, node_par, t_start, t_end, h_step, scale, status, counters, a);
   if (rc != GAL_OK) return rc;
   if (n_subset < 0 || n_subset > n || (subset == nullptr && n_subset != n)) return GAL_E_INVALID;
   if (n_subset == 0) return GAL_OK;
   a.n_claim = n_subset;
+  a.n_claim_dev = n_subset_dev;   // when set, n_subset is the list's capacity
   a.order = subset;
   if (h->flags & GAL_FLAG_COST_ORDER) {
     const int threads = 256;
     hipLaunchKernelGGL(cost_probe_kernel, dim3((n_subset + threads - 1) / threads), dim3(threads), 0, s, n_subset,
                        subset, ld, (const double*)y, node_par, t_start, t_end, (const double*)scale,
                        (const double*)h->d_params, (const double*)h->d_table, h->D, h->cd, a.mode,
-                       (h->flags & GAL_FLAG_SCALES_PROVIDED) ? 1 : 0, h->d_key, h->d_index);
+                       (h->flags & GAL_FLAG_SCALES_PROVIDED) ? 1 : 0, n_subset_dev, h->d_key, h->d_index);
     HIP_TRY(hipGetLastError());
     size_t tb = h->sort_temp_bytes;
     HIP_TRY(hipcub::DeviceRadixSort::SortPairsDescending(h->d_sort_temp, tb, h->d_key, h->d_key_sorted, h->d_index,

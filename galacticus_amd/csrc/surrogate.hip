@@ -167,6 +167,9 @@ __global__ __launch_bounds__(256) void gemm_kernel(GemmArgs g) {
 }
 
 // ---- training support kernels ----------------------------------------------
+// Network outputs enter sinh(); beyond +-20 they are clipped (sinh(20) = 2.4e8).
+constexpr float kOutClip = 20.0f;
+
 struct LossArgs {
   const float* out;
   const float* target;
@@ -191,7 +194,10 @@ __global__ __launch_bounds__(256) void loss_kernel(LossArgs a) {
       const float res = o[v] - t[v];
       mse += a.w[v] * res * res;
       d[v] = f2bf(2.0f * a.w[v] * res * a.inv_wsum * a.inv_B);
-      if ((a.head_mask >> v) & 1u) emax = fmaxf(emax, fabsf(res) * a.sigma[v]);
+      // targets are asinh((r - m)/sigma) of the residual r: the error of the
+      // residual itself is sigma |sinh(o) - sinh(t)|
+      if ((a.head_mask >> v) & 1u)
+        emax = fmaxf(emax, a.sigma[v] * fabsf(sinhf(fminf(fmaxf(o[v], -kOutClip), kOutClip)) - sinhf(t[v])));
     }
     // a sample whose target was clipped carries a flag in column 31: its true
     // error is at least 1 (asinh units), so the head learns to reject it
@@ -480,7 +486,7 @@ __global__ __launch_bounds__(256) void encode_kernel(int n, int ld, const double
         d = asinh(b / s) - asinh(a / s) - du[v];
       }
       float z = (float)d;
-      if (norm) z = (z - norm[2 * kIn + v]) / norm[3 * kIn + v];
+      if (norm) z = (float)asinh((d - (double)norm[2 * kIn + v]) / (double)norm[3 * kIn + v]);
       tg[v] = z;
     }
     tg[GAL_MLP_HEAD] = 0.0f;
@@ -533,7 +539,8 @@ __global__ __launch_bounds__(256) void apply_kernel(GateArgs a) {
 #pragma unroll
       for (int v = 0; v < kNV; ++v) {
         const int role = var_role(v);
-        const double d = du[v] + (double)nm[2 * kIn + v] + (double)nm[3 * kIn + v] * (double)o[v];
+        const double ov = fmin(fmax((double)o[v], -(double)kOutClip), (double)kOutClip);
+        const double d = du[v] + (double)nm[2 * kIn + v] + (double)nm[3 * kIn + v] * sinh(ov);
         if (role == R_CONST) {
           y1[v] = y0[v];
         } else if (role == R_SPIN) {
@@ -544,6 +551,11 @@ __global__ __launch_bounds__(256) void apply_kernel(GateArgs a) {
           const double s = var_scale(f, role);
           y1[v] = s * sinh(asinh(y0[v] / s) + d);
         }
+        // a small negative value of a non-negative quantity is set to zero,
+        // as the reference's post-step fixes do after an integrator step
+        if (var_non_negative(v) && y1[v] < 0.0 && role != R_ENERGY &&
+            y1[v] >= -a.gate.negative_tolerance * var_scale(f, role))
+          y1[v] = 0.0;
         ok = ok && isfinite(y1[v]);
         ok = ok && !(var_non_negative(v) && y1[v] < 0.0);
       }
@@ -1104,13 +1116,18 @@ int gal_surrogate_evolve_d(gal_ode_batch* h, gal_mlp* m, const float* norm, cons
     GAL_TRY(apply_chunk(h, gate, nc, ld, y + c0, node_par + c0, t_start + c0, t_end + c0, m->d_iout, norm, sc,
                         status + c0, accepted + c0, m->d_rej, m->d_rej_count, c0, s));
   }
-  int n_rej = 0;
-  HIP_TRY(hipMemcpyAsync(&n_rej, m->d_rej_count, sizeof(int), hipMemcpyDeviceToHost, s));
-  HIP_TRY(hipStreamSynchronize(s));
-  if (n_rejected) *n_rejected = n_rej;
-  if (n_rej == 0) return GAL_OK;
-  return gal_ode_evolve_subset(h, n, ld, y, node_par, t_start, t_end, h_step, scale, status, nullptr, m->d_rej, n_rej,
-                               s);
+  // The integrator takes the rejected list with its count in device memory,
+  // so nothing here waits for the GPU: the call only enqueues work on s
+  // unless the caller asks for the count on the host.
+  GAL_TRY(gal_ode_evolve_subset(h, n, ld, y, node_par, t_start, t_end, h_step, scale, status, nullptr, m->d_rej, n,
+                                s, m->d_rej_count));
+  if (n_rejected) {
+    int n_rej = 0;
+    HIP_TRY(hipMemcpyAsync(&n_rej, m->d_rej_count, sizeof(int), hipMemcpyDeviceToHost, s));
+    HIP_TRY(hipStreamSynchronize(s));
+    *n_rejected = n_rej;
+  }
+  return GAL_OK;
 }
 
 }  // extern "C"

@@ -77,9 +77,11 @@ class GateConfig:
     mass_tolerance: float = 5.0e-3
     metal_tolerance: float = 2.0e-2
     bh_slack: float = 20.0
+    negative_tolerance: float = 1.0e-2   # small negatives (in units of the asinh scale) become zero
 
     def c_struct(self):
-        return _lib.Gate(math.log(self.error_max), self.mass_tolerance, self.metal_tolerance, self.bh_slack)
+        return _lib.Gate(math.log(self.error_max), self.mass_tolerance, self.metal_tolerance, self.bh_slack,
+                         self.negative_tolerance)
 
 
 class SurrogateMLP:
@@ -246,27 +248,35 @@ def encode(evolver: NodeEvolverBatched, pairs: PairSet, norm=None, raw=False, wi
 
 
 def fit_normalization(x_raw, t_raw, sample=2_000_000):
-    """Standardisation (4 x 32 fp32): input and target means and standard
-    deviations (from at most `sample` rows)."""
+    """Normalisation (4 x 32 fp32) from at most `sample` rows: input means
+    and standard deviations; for each residual target its median m and a
+    robust scale sigma, 1.4826 x the median absolute deviation of the rows
+    that deviate from m at all (a component that is empty in most rows still
+    gets the scale of its active rows).  The network learns
+    asinh((r - m) / sigma): unit scale for typical rows, logarithmic tails."""
     import torch
     xs = x_raw[:sample].double()
     ts = t_raw[:sample].double()
     norm = torch.zeros((4, 32), dtype=torch.float32, device=x_raw.device)
     norm[0] = xs.mean(0).float()
     norm[1] = xs.std(0).float()
-    norm[2] = ts.mean(0).float()
-    norm[3] = ts.std(0).float()
     norm[1] = torch.where(norm[1] > 1e-6, norm[1], torch.ones_like(norm[1]))
-    untrained = [v for v in range(32) if v not in TRAINED_VARS]
-    norm[2, untrained] = 0.0
-    norm[3, untrained] = 1.0
-    norm[3] = torch.where(norm[3] > 1e-9, norm[3], torch.ones_like(norm[3]))
+    med = ts.median(dim=0).values
+    dev = (ts - med).abs()
+    for v in range(32):
+        if v not in TRAINED_VARS:
+            norm[2, v], norm[3, v] = 0.0, 1.0
+            continue
+        active = dev[:, v][dev[:, v] > 1e-9]
+        norm[2, v] = float(med[v])
+        norm[3, v] = max(1.4826 * float(active.median()), 1e-6) if active.numel() else 1.0
     return norm.contiguous()
 
 
-def clip_targets(target, clip=8.0):
-    """Clip standardised targets to +-clip and flag clipped rows in column 31
-    (their error counts as large for the error head)."""
+def clip_targets(target, clip=16.0):
+    """Clip targets to +-clip (asinh units: residuals beyond sigma sinh(16) =
+    4.4e6 sigma) and flag clipped rows in column 31 (their error counts as
+    large for the error head)."""
     import torch
     big = (target[:, :30].abs() > clip).any(dim=1)
     target[:, :30].clamp_(-clip, clip)
@@ -340,8 +350,10 @@ class SurrogateEvolver:
         self.gate = gate or GateConfig()
 
     def evolve_device(self, y, node_par, t_start, t_end, h_step, scale=None, status=None, accepted=None,
-                      stream=None):
-        """Returns (status, accepted, n_rejected)."""
+                      stream=None, count_rejected=True):
+        """Returns (status, accepted, n_rejected).  With count_rejected=False
+        the call only enqueues work on the stream and n_rejected is None
+        (count ``accepted == 0`` once the stream is done)."""
         import torch
         n = int(t_start.shape[0])
         ld = int(y.shape[1])
@@ -355,6 +367,7 @@ class SurrogateEvolver:
         g = self.gate.c_struct()
         _lib.check(self.evolver.L.gal_surrogate_evolve_d(
             self.evolver.handle, self.mlp.handle, _ptr(self.norm), ctypes.byref(g), n, ld, _ptr(y), _ptr(node_par),
-            _ptr(t_start), _ptr(t_end), _ptr(h_step), _ptr(scale), _ptr(status), _ptr(accepted), ctypes.byref(n_rej),
+            _ptr(t_start), _ptr(t_end), _ptr(h_step), _ptr(scale), _ptr(status), _ptr(accepted),
+            ctypes.byref(n_rej) if count_rejected else None,
             _stream(stream)), "gal_surrogate_evolve_d")
-        return status, accepted, n_rej.value
+        return status, accepted, (n_rej.value if count_rejected else None)

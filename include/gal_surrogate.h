@@ -66,6 +66,10 @@ typedef struct gal_surrogate_gate {
   double mass_tolerance;  /* |baryon budget residual| / (M_b0 + accreted)        */
   double metal_tolerance; /* |metal budget residual| / (Z0 + yield * dM_formed)  */
   double bh_slack;        /* extra negative slack per unit black-hole growth     */
+  double negative_tolerance; /* a decoded non-negative quantity in [-tol s, 0)
+                                (s its asinh scale) is set to 0, as the
+                                reference's post-step fixes do; below it the
+                                step is rejected                               */
 } gal_surrogate_gate;
 
 /* ---- the network ------------------------------------------------------- */
@@ -134,7 +138,11 @@ int gal_surrogate_apply_d(gal_ode_batch* h, const gal_surrogate_gate* gate, int 
 /* The surrogate evolve: encode -> network -> gate for all n nodes, then the
    FP64 integrator (gal_ode_batch_evolve_d semantics) on the rejected ones.
    accepted (device int n) reports the route of each node; *n_rejected
-   (host) the number evolved by the integrator. */
+   (host) the number evolved by the integrator.  With n_rejected NULL the
+   call only enqueues work on stream (the integrator reads the rejected
+   count from device memory); otherwise it waits for the stream to read it.
+   One network handle serves one stream at a time (it holds the inference
+   buffers and the rejected list). */
 int gal_surrogate_evolve_d(gal_ode_batch* h, gal_mlp* m, const float* norm, const gal_surrogate_gate* gate,
                            int n, int ld, double* y, const double* node_par, const double* t_start,
                            const double* t_end, double* h_step, double* scale, int* status, int* accepted,

@@ -23,6 +23,7 @@ import math
 import numpy as np
 
 N_IN, N_HID, N_OUT, OUT_LD, HEAD = 32, 256, 31, 32, 30
+OUT_CLIP = 20.0   # network outputs enter sinh() clipped to +-20
 FAN_IN = (N_IN, N_HID, N_HID, N_HID)
 FAN_OUT = (N_HID, N_HID, N_HID, N_OUT)
 
@@ -101,7 +102,9 @@ def loss_grad(out, target, w, sigma, head_mask, head_w):
     wsum = np.float32(w.sum())
     mse = (w * r * r).sum(axis=1)
     hm = np.asarray(head_mask, dtype=bool)
-    e = (np.abs(r[:, hm]) * np.asarray(sigma, dtype=np.float32)[hm]).max(axis=1)
+    o_c = np.clip(out[:, :30], -OUT_CLIP, OUT_CLIP).astype(np.float32)
+    err = np.abs(np.sinh(o_c) - np.sinh(target[:, :30].astype(np.float32))).astype(np.float32)
+    e = (err[:, hm] * np.asarray(sigma, dtype=np.float32)[hm]).max(axis=1)
     e = np.where(target[:, 31] != 0, np.maximum(e, np.float32(1.0)), e)   # clipped samples
     rh = out[:, HEAD] - np.log(e + np.float32(1e-7)).astype(np.float32)
     loss = float((mse / wsum + np.float32(head_w) * rh * rh).sum() / n)
@@ -141,6 +144,21 @@ def forward64(p, x):
     return out, cache
 
 
+def residual_error(out, target, sigma, hm):
+    """Worst error of the head variables' residuals (float64): targets are
+    asinh((r - m)/sigma), so the residual error is sigma |sinh(o) - sinh(t)|."""
+    o = np.clip(out[:, :30], -OUT_CLIP, OUT_CLIP)
+    err = np.abs(np.sinh(o) - np.sinh(target[:, :30])) * np.asarray(sigma)
+    return err[:, hm].max(axis=1)
+
+
+def network_increment(out, norm):
+    """Residual increments (asinh units of the state) from network outputs:
+    m + sigma sinh(o), with the outputs clipped to +-OUT_CLIP."""
+    o = np.clip(np.asarray(out, dtype=np.float64)[..., :30], -OUT_CLIP, OUT_CLIP)
+    return np.asarray(norm, dtype=np.float64)[2, :30] + np.asarray(norm, dtype=np.float64)[3, :30] * np.sinh(o)
+
+
 def loss64(out, target, w, sigma, head_mask, head_w, e_fixed=None):
     """Float64 loss; the head target ln(e) is a constant of the step (no
     gradient flows through it), so finite differences hold it at e_fixed."""
@@ -149,7 +167,7 @@ def loss64(out, target, w, sigma, head_mask, head_w, e_fixed=None):
     mse = (w * r * r).sum(axis=1) / w.sum()
     hm = np.asarray(head_mask, dtype=bool)
     if e_fixed is None:
-        e = (np.abs(r[:, hm]) * np.asarray(sigma)[hm]).max(axis=1)
+        e = residual_error(out, target, sigma, hm)
         e = np.where(target[:, 31] != 0, np.maximum(e, 1.0), e)          # clipped samples
     else:
         e = e_fixed
@@ -162,7 +180,7 @@ def backward64(p, cache, out, target, w, sigma, head_mask, head_w):
     n = out.shape[0]
     r = out[:, :30] - target[:, :30]
     hm = np.asarray(head_mask, dtype=bool)
-    e = (np.abs(r[:, hm]) * np.asarray(sigma)[hm]).max(axis=1)
+    e = residual_error(out, target, sigma, hm)
     e = np.where(target[:, 31] != 0, np.maximum(e, 1.0), e)              # clipped samples
     rh = out[:, HEAD] - np.log(e + 1e-7)
     d = np.zeros((n, N_OUT))
@@ -204,6 +222,9 @@ VAR_ROLES = [
     ROLE_BH, ROLE_UNIT, ROLE_CONST, ROLE_SPIN, ROLE_CONST,                                        # 25-29
 ]
 INPUT_VARS = [0, 1, 2, 3, 4, 5, 6, 7, 12, 13, 15, 16, 17, 18, 19, 20, 21, 22, 23, 24, 25, 26]
+# every state variable but the angular momenta that can change sign, the
+# radiated energy, the black-hole radial position and the spin angular momentum
+NON_NEGATIVE = frozenset(v for v in range(30) if v not in (2, 4, 11, 27, 28))
 
 
 def node_frame(mass, r_vir, v_vir, f_baryon, unit_energy):
@@ -274,8 +295,10 @@ def _sinh(x):
         return math.copysign(math.inf, x)
 
 
-def decode(y0, d, dt, halo, node_par, f_baryon, unit_energy, rate=None):
-    """Inverse of the target map: state at t1 from y0 and raw residuals d."""
+def decode(y0, d, dt, halo, node_par, f_baryon, unit_energy, rate=None, negative_tolerance=0.0):
+    """Inverse of the target map: state at t1 from y0 and raw residuals d.
+    A non-negative quantity that decodes into [-negative_tolerance s, 0) is
+    set to zero."""
     sc = node_frame(halo["mass"], halo["r_vir"], halo["v_vir"], f_baryon, unit_energy)
     du = _baseline(y0, np.zeros(30) if rate is None else rate, dt, sc)
     y1 = np.array(y0, dtype=np.float64)
@@ -289,4 +312,6 @@ def decode(y0, d, dt, halo, node_par, f_baryon, unit_energy, rate=None):
             y1[v] = y0[v] + sc[role] * _sinh(du[v] + d[v])
         else:
             y1[v] = sc[role] * _sinh(math.asinh(y0[v] / sc[role]) + du[v] + d[v])
+            if v in NON_NEGATIVE and -negative_tolerance * sc[role] <= y1[v] < 0.0:
+                y1[v] = 0.0
     return y1
