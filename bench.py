@@ -279,20 +279,20 @@ def chained_histories(ev, se, physics, args, n=20000):
             "rk_seconds": times["rk"], "surrogate_seconds": times["surrogate"]}
 
 
-def trees_leg(physics, table, tol_abs, tol_rel, args):
-    """64 Parkinson-Cole-Helly trees (seed 0), roots log-uniform in 1e10-1e12
-    Msun at z=0, resolution 1e9 Msun, evolved from z=5 in the reference's walk
-    order, wave by wave."""
+def _forest_run(physics, table, tol_abs, tol_rel, n_trees, log_m, z_start, label, cpu_threads, cpu=True):
+    """Build a PCH forest, record its walk, evolve it wave by wave on the GPU
+    and (cpu) with the CPU oracle's tree runner over cpu_threads threads;
+    compare the z=0 stellar masses of the roots."""
     import torch
     from galacticus_amd import trees as tr, workloads as wl
     from galacticus_amd.evolver import NodeEvolverBatched
     cosmo = tr.cosmology(physics)
     t0 = float(wl.scale_factor_to_cosmic_time(1.0))
-    t_z5 = float(wl.scale_factor_to_cosmic_time(1.0 / 6.0))
+    t_start = float(wl.scale_factor_to_cosmic_time(1.0 / (1.0 + z_start)))
     rng = np.random.Generator(np.random.PCG64(0))
-    roots = 10.0 ** rng.uniform(10.0, 12.0, 64)
+    roots = 10.0 ** rng.uniform(log_m[0], log_m[1], n_trees)
     t = time.perf_counter()
-    forest = tr.Forest.build_pch(64, roots, t0, cosmo, tr.pch_params(1e9, t_z5), seed=0)
+    forest = tr.Forest.build_pch(n_trees, roots, t0, cosmo, tr.pch_params(1e9, t_start), seed=0)
     build_s = time.perf_counter() - t
     wp = tr.walk_params(t0)
     t = time.perf_counter()
@@ -308,16 +308,59 @@ def trees_leg(physics, table, tol_abs, tol_rel, args):
         holder["y"] = y
     gpu_s, _ = _timed(run)
     st = holder["st"].cpu().numpy()
+    yg = holder["y"].cpu().numpy()
     gyr = sched.node_gyr()
     k = np.bincount(sched.ops["kind"], minlength=4)
-    log(f"trees: {forest.n_nodes} nodes, {sched.n_ops} operations in {sched.n_waves} waves, GPU {gpu_s:.3f} s "
-        f"({gyr / gpu_s:.4g} node-Gyr/s)")
-    return {"workload": "64 PCH trees, seed 0, roots log-uniform 1e10-1e12 Msun, resolution 1e9 Msun, z=5 -> 0",
-            "nodes": forest.n_nodes, "operations": sched.n_ops, "waves": sched.n_waves,
-            "evolve_intervals": int(k[0]), "promotions": int(k[1]), "halo_mergers": int(k[2]),
-            "galaxy_mergers": int(k[3]), "node_gyr": gyr, "gpu_seconds": gpu_s, "node_gyr_per_s": gyr / gpu_s,
-            "host_build_seconds": build_s, "host_schedule_seconds": sched_s,
-            "statuses_ok": bool((st == 0).all())}
+    out = {"workload": label, "nodes": forest.n_nodes, "operations": sched.n_ops, "waves": sched.n_waves,
+           "evolve_intervals": int(k[0]), "promotions": int(k[1]), "halo_mergers": int(k[2]),
+           "galaxy_mergers": int(k[3]), "node_gyr": gyr, "gpu_seconds": gpu_s, "node_gyr_per_s": gyr / gpu_s,
+           "host_build_seconds": build_s, "host_schedule_seconds": sched_s, "statuses_ok": bool((st == 0).all())}
+    log(f"trees ({label}): {forest.n_nodes} nodes, {sched.n_ops} operations in {sched.n_waves} waves, "
+        f"GPU {gpu_s:.3f} s ({gyr / gpu_s:.4g} node-Gyr/s)")
+    del holder, ev
+    if cpu:
+        import oracle
+        from galacticus_amd import params as gp
+        t = time.perf_counter()
+        yc, _, stc, bad = oracle.tree_run(sched.ops, y0, -np.ones(forest.n_nodes), forest.tree_of_node(),
+                                          forest.n_trees, physics, table, tol_abs, tol_rel, gp.FLAG_REUSE_STEP_SIZE,
+                                          0.25, root_tol_rel=1.0e-6, nthreads=cpu_threads)
+        cpu_s = time.perf_counter() - t
+        from galacticus_amd.workloads import V
+        r = forest.roots()
+        mg = yg[V["DISK_MASS_STELLAR"], r] + yg[V["SPHEROID_MASS_STELLAR"], r]
+        mc = yc[V["DISK_MASS_STELLAR"], r] + yc[V["SPHEROID_MASS_STELLAR"], r]
+        rel = np.abs(mg - mc) / np.maximum(mc, 1e-300)
+        bins = np.arange(6.0, 12.51, 0.25)
+        hg = np.histogram(np.log10(np.maximum(mg, 1e-30)), bins)[0]
+        hc = np.histogram(np.log10(np.maximum(mc, 1e-30)), bins)[0]
+        smf_diff = int(np.abs(hg - hc).sum())
+        out["cpu_baseline"] = {"value": gyr / cpu_s, "unit": "node-Gyr/s", "cores": cpu_threads, "kind": "port",
+                               "seconds": cpu_s, "failed_evolves": int(bad),
+                               "sample": "the whole forest, oracle/oracle.c orc_tree_run (trees in parallel, "
+                                         "operations in schedule order), cooling-radius root tolerance 1e-6"}
+        out["gpu_over_cpu"] = cpu_s / gpu_s
+        out["z0_stellar_mass_vs_cpu"] = {"median_relative": float(np.median(rel)), "max_relative": float(rel.max()),
+                                         "smf_bins_0.25dex_count_moved": smf_diff, "roots": int(len(r))}
+        log(f"trees ({label}): CPU {cpu_s:.2f} s on {cpu_threads} threads, GPU/CPU {cpu_s / gpu_s:.1f}x, "
+            f"root M* median rel diff {np.median(rel):.2e}, SMF galaxies moved between bins {smf_diff}")
+    return out
+
+
+def trees_leg(physics, table, tol_abs, tol_rel, args):
+    """Merger trees in the reference's walk order, wave by wave: the 64-tree
+    workload (roots 1e10-1e12 Msun, z=5 -> 0) and a larger forest (roots
+    1e11-1e13 Msun, z=8 -> 0), each against the CPU oracle's tree runner."""
+    threads = min(16, os.cpu_count() or 1)
+    small = _forest_run(physics, table, tol_abs, tol_rel, 64, (10.0, 12.0), 5.0,
+                        "64 PCH trees, seed 0, roots log-uniform 1e10-1e12 Msun, resolution 1e9 Msun, z=5 -> 0",
+                        threads, cpu=not args.no_cpu)
+    out = dict(small)
+    if args.large_trees > 0:
+        out["large"] = _forest_run(physics, table, tol_abs, tol_rel, args.large_trees, (11.0, 13.0), 8.0,
+                                   f"{args.large_trees} PCH trees, seed 0, roots log-uniform 1e11-1e13 Msun, "
+                                   f"resolution 1e9 Msun, z=8 -> 0", threads, cpu=not args.no_cpu)
+    return out
 
 
 def main():
@@ -335,6 +378,8 @@ def main():
     ap.add_argument("--cpu-seconds", type=float, default=15.0)
     ap.add_argument("--no-surrogate", action="store_true", help="skip the surrogate training/inference legs")
     ap.add_argument("--no-trees", action="store_true", help="skip the merger-tree leg")
+    ap.add_argument("--large-trees", type=int, default=256,
+                    help="trees in the larger forest of the tree leg (0 skips it)")
     ap.add_argument("--train-pairs", type=int, default=4000000)
     ap.add_argument("--train-epochs", type=int, default=20)
     ap.add_argument("--eval-pairs", type=int, default=1000000)

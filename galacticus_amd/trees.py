@@ -233,6 +233,11 @@ class Forest:
     def roots(self):
         return self.tree_offset[:-1]
 
+    def tree_of_node(self):
+        """Index of the tree that holds each node (nodes of a tree are
+        contiguous, tree_offset[t] <= node < tree_offset[t+1])."""
+        return (np.searchsorted(self.tree_offset, np.arange(self.n_nodes), side="right") - 1).astype(np.int32)
+
     def initial_state(self, cosmo: Cosmology, walk: WalkParams, metal_fraction=1e-4, black_hole_seed=100.0):
         y = np.zeros((gp.NVAR, self.n_nodes))
         _lib.check(self.L.gal_forest_initial_state(self.handle, ctypes.byref(cosmo), ctypes.byref(walk),

@@ -72,6 +72,10 @@ def lib():
         L.orc_test_ode_solve.argtypes = [ctypes.c_int, ctypes.c_double, ctypes.c_double, dp, ctypes.c_double,
                                          ctypes.c_double, dp, lp]
         L.orc_test_ode_solve.restype = ctypes.c_int
+        L.orc_tree_run.argtypes = [ctypes.c_int64, ip, ip, ip, dp, dp, dp, ctypes.c_int, ip, ctypes.c_int, dp, dp, ip,
+                                   dp, cp, ctypes.c_double, ctypes.c_double, ctypes.c_int, ctypes.c_double,
+                                   ctypes.c_double, ctypes.c_int]
+        L.orc_tree_run.restype = ctypes.c_int
         _lib = L
     return _lib
 
@@ -181,3 +185,29 @@ def test_ode_solve(which, x0, x1, y0, tol_abs, tol_rel):
     cnt = np.zeros(3, dtype=np.int64)
     st = lib().orc_test_ode_solve(which, x0, x1, _dp(y), tol_abs, tol_rel, _dp(h), _lp(cnt))
     return st, y, float(h[0]), cnt
+
+
+def tree_run(ops, y0, h0, tree_of_node, n_trees, params, table, tol_abs, tol_rel, flags, major_ratio=0.25,
+             root_tol_rel=1.0e-14, nthreads=1):
+    """Apply a schedule's operations (structured array with the fields of
+    gal_schedule_op) on the CPU, one at a time per tree, trees in parallel.
+    Returns (y, h, status, n_failed)."""
+    L = lib()
+    y = np.array(y0, dtype=np.float64, order="C", copy=True)
+    ld = y.shape[1]
+    h = np.array(h0, dtype=np.float64, copy=True)
+    status = np.zeros(ld, dtype=np.int32)
+    kind = np.ascontiguousarray(ops["kind"], dtype=np.int32)
+    a = np.ascontiguousarray(ops["a"], dtype=np.int32)
+    b = np.ascontiguousarray(ops["b"], dtype=np.int32)
+    t0 = np.ascontiguousarray(ops["t0"], dtype=np.float64)
+    t1 = np.ascontiguousarray(ops["t1"], dtype=np.float64)
+    npar = np.ascontiguousarray(ops["node_par"], dtype=np.float64)
+    tree = np.ascontiguousarray(tree_of_node, dtype=np.int32)
+    cs = cooling_struct(table)
+    bad = L.orc_tree_run(len(kind), _ip(kind), _ip(a), _ip(b), _dp(t0), _dp(t1), _dp(npar), ld, _ip(tree), n_trees,
+                         _dp(y), _dp(h), _ip(status), _dp(np.ascontiguousarray(params, dtype=np.float64)),
+                         ctypes.byref(cs), tol_abs, tol_rel, flags, root_tol_rel, major_ratio, nthreads)
+    if bad < 0:
+        raise RuntimeError(f"orc_tree_run returned {bad}")
+    return y, h, status, bad

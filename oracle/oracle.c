@@ -29,6 +29,7 @@
  * written.
  */
 #include "oracle.h"
+#include "../include/gal_tree.h"
 #include <math.h>
 #include <string.h>
 #include <float.h>
@@ -1123,6 +1124,104 @@ int orc_evolve_batch(int n, int ld, double *y, const double *np, const double *t
     if (counters) { counters[i] = cnt[0]; counters[(size_t)n + i] = cnt[1]; counters[2 * (size_t)n + i] = cnt[2]; }
     if (st != ST_SUCCESS) nbad++;
   }
+  return nbad;
+}
+
+/* ---------------------------------------------------------------------- */
+/* Tree schedules (the sequential restatement of oracle/trees.py
+   run_sequential + merge_galaxy_simple, which follow the walk of
+   merger_trees/evolver/standard.F90:291-1035 and the mass movements of
+   satellites/merging/mass_movements/simple.F90:208-290). */
+static void merge_galaxy_simple_c(double *y, int ld, int sat, int host, double major_ratio)
+{
+#define Y(v, s) y[(size_t)(v) * ld + (s)]
+  const double ms = Y(GAL_V_DISK_MASS_GAS, sat) + Y(GAL_V_DISK_MASS_STELLAR, sat) +
+                    Y(GAL_V_SPHEROID_MASS_GAS, sat) + Y(GAL_V_SPHEROID_MASS_STELLAR, sat);
+  const double mh = Y(GAL_V_DISK_MASS_GAS, host) + Y(GAL_V_DISK_MASS_STELLAR, host) +
+                    Y(GAL_V_SPHEROID_MASS_GAS, host) + Y(GAL_V_SPHEROID_MASS_STELLAR, host);
+  const int major = ms > 0.0 && mh > 0.0 && fmin(ms, mh) >= major_ratio * fmax(ms, mh);
+  static const int to_sph[6][2] = {
+      {GAL_V_DISK_MASS_GAS, GAL_V_SPHEROID_MASS_GAS}, {GAL_V_DISK_ABUNDANCES_GAS, GAL_V_SPHEROID_ABUNDANCES_GAS},
+      {GAL_V_DISK_MASS_STELLAR, GAL_V_SPHEROID_MASS_STELLAR},
+      {GAL_V_DISK_ABUNDANCES_STELLAR, GAL_V_SPHEROID_ABUNDANCES_STELLAR},
+      {GAL_V_DISK_MASS_STELLAR_FORMED, GAL_V_SPHEROID_MASS_STELLAR_FORMED},
+      {GAL_V_DISK_ANGULAR_MOMENTUM, GAL_V_SPHEROID_ANGULAR_MOMENTUM}};
+  static const int same[16] = {
+      GAL_V_SPHEROID_MASS_GAS, GAL_V_SPHEROID_ABUNDANCES_GAS, GAL_V_SPHEROID_MASS_STELLAR,
+      GAL_V_SPHEROID_ABUNDANCES_STELLAR, GAL_V_SPHEROID_MASS_STELLAR_FORMED, GAL_V_SPHEROID_ANGULAR_MOMENTUM,
+      GAL_V_HOT_MASS, GAL_V_HOT_ABUNDANCES, GAL_V_HOT_ANGULAR_MOMENTUM, GAL_V_HOT_OUTFLOWED_MASS,
+      GAL_V_HOT_OUTFLOWED_ANGULAR_MOMENTUM, GAL_V_HOT_OUTFLOWED_ABUNDANCES, GAL_V_HOT_UNACCRETED_MASS,
+      GAL_V_HOT_UNACCRETED_ABUNDANCES, GAL_V_HOT_ENERGY_RADIATED, GAL_V_BH_MASS};
+  for (int k = 0; k < 6; k++) {
+    Y(to_sph[k][1], host) += Y(to_sph[k][0], sat);
+    Y(to_sph[k][0], sat) = 0.0;
+  }
+  for (int k = 0; k < 16; k++) {
+    Y(same[k], host) += Y(same[k], sat);
+    Y(same[k], sat) = 0.0;
+  }
+  if (major) {
+    for (int k = 0; k < 6; k++) {
+      Y(to_sph[k][1], host) += Y(to_sph[k][0], host);
+      Y(to_sph[k][0], host) = 0.0;
+    }
+  }
+#undef Y
+}
+
+int orc_tree_run(int64_t n_ops, const int *kind, const int *a, const int *b, const double *t0, const double *t1,
+                 const double *node_par, int ld, const int *tree_of_node, int n_trees, double *y, double *h,
+                 int *status, const double *params, const orc_cooling *table, double tol_abs, double tol_rel,
+                 int flags, double root_tol_rel, double major_ratio, int nthreads)
+{
+  if (n_ops < 0 || ld < 1 || n_trees < 1) return GAL_E_INVALID;
+  for (int64_t k = 0; k < n_ops; k++) {
+    if (a[k] < 0 || a[k] >= ld || b[k] >= ld) return GAL_E_INVALID;
+    const int t = tree_of_node[a[k]];
+    if (t < 0 || t >= n_trees) return GAL_E_INVALID;
+  }
+  /* operations of each tree, in schedule order */
+  int64_t *start = calloc((size_t)n_trees + 1, sizeof(int64_t));
+  int64_t *list = malloc(sizeof(int64_t) * (size_t)(n_ops > 0 ? n_ops : 1));
+  int64_t *fill = malloc(sizeof(int64_t) * (size_t)n_trees);
+  if (!start || !list || !fill) { free(start); free(list); free(fill); return GAL_E_NOMEM; }
+  for (int64_t k = 0; k < n_ops; k++) start[tree_of_node[a[k]] + 1]++;
+  for (int t = 0; t < n_trees; t++) start[t + 1] += start[t];
+  for (int t = 0; t < n_trees; t++) fill[t] = start[t];
+  for (int64_t k = 0; k < n_ops; k++) list[fill[tree_of_node[a[k]]]++] = k;
+  int nbad = 0;
+#ifdef _OPENMP
+  if (nthreads < 1) nthreads = 1;
+#pragma omp parallel for schedule(dynamic, 1) num_threads(nthreads) reduction(+:nbad)
+#endif
+  for (int t = 0; t < n_trees; t++) {
+    for (int64_t j = start[t]; j < start[t + 1]; j++) {
+      const int64_t k = list[j];
+      const int na = a[k];
+      if (kind[k] == GAL_OP_EVOLVE) {
+        orc_sys s;
+        memset(&s, 0, sizeof(s));
+        s.P = params; s.ct = table; s.eps_abs = tol_abs; s.eps_rel = tol_rel; s.root_tol = root_tol_rel;
+        double yr[NV];
+        for (int v = 0; v < NV; v++) yr[v] = y[(size_t)v * ld + na];
+        for (int p = 0; p < GAL_NPARAM_NODE; p++) s.np[p] = node_par[(size_t)GAL_NPARAM_NODE * k + p];
+        scales_compute(params, s.np, t0[k], yr, s.scale);
+        if (flags & GAL_FLAG_ENFORCE_NON_NEGATIVITY) nonneg_flags(s.nonneg);
+        double hn = h[na];
+        const int st = node_evolve(&s, t0[k], t1[k], yr, &hn, flags, NULL);
+        for (int v = 0; v < NV; v++) y[(size_t)v * ld + na] = yr[v];
+        h[na] = hn;
+        status[na] = st;
+        if (st != ST_SUCCESS) nbad++;
+      } else if (kind[k] == GAL_OP_PROMOTE) {
+        for (int v = 0; v < NV; v++) y[(size_t)v * ld + b[k]] = y[(size_t)v * ld + na];
+        h[b[k]] = h[na];
+      } else if (kind[k] == GAL_OP_MERGE_GALAXY) {
+        merge_galaxy_simple_c(y, ld, na, b[k], major_ratio);
+      }
+    }
+  }
+  free(start); free(list); free(fill);
   return nbad;
 }
 

@@ -69,6 +69,24 @@ void orc_halo_properties(const double *params, const orc_cooling *table,
                          const double *y_row, double root_tol_rel,
                          double out[8]);
 
+/* Apply a tree schedule on the CPU: operations one at a time in schedule
+   order within each tree (trees are independent and run in parallel over
+   nthreads).  kind/a/b/t0/t1/node_par[5*op] as gal_schedule_op of
+   include/gal_tree.h; tree_of_node[ld] names each node's tree (0..n_trees-1).
+   Evolve: orc_evolve of node a over [t0, t1] with step size h[a]; promote:
+   y[:, b] = y[:, a], h[b] = h[a]; galaxy merger: the simple mass movements
+   (satellite a into host b, spheroid destinations, major if the smaller
+   galaxy is at least major_ratio of the larger); becoming a satellite moves
+   no mass.  status[a] is the last evolve's status.  Returns the number of
+   non-success evolves.                                                      */
+int orc_tree_run(int64_t n_ops, const int *kind, const int *a, const int *b,
+                 const double *t0, const double *t1, const double *node_par,
+                 int ld, const int *tree_of_node, int n_trees, double *y,
+                 double *h, int *status, const double *params,
+                 const orc_cooling *table, double tol_abs, double tol_rel,
+                 int flags, double root_tol_rel, double major_ratio,
+                 int nthreads);
+
 /* Generic fixed-dimension Cash-Karp solve of a built-in test system used to
    pin the integrator against closed forms (source/tests/ODE_solver.F90:59-104).
    which: 1 = y'=sin(x) ; 2 = y''=-y (y1'=y2, y2'=-y1).                   */

@@ -223,3 +223,32 @@ def test_fully_specified_reader():
 def test_forest_from_arrays_rejects_bad_links():
     with pytest.raises(Exception):
         tr.Forest.from_arrays([0, 2], [-1, 5], [1, -1], [-1, -1], [1e12, 1e12], [13.47, 12.47])
+
+
+def test_c_tree_runner_equals_sequential_restatement(cosmo, cooling_table):
+    """oracle.tree_run (C, trees in parallel) applies a schedule exactly as the
+    Python restatement does one operation at a time: identical states."""
+    import oracle
+    from galacticus_amd import params as gp
+    physics = gp.physics_params()
+    t0 = float(wl.scale_factor_to_cosmic_time(1.0))
+    t5 = float(wl.scale_factor_to_cosmic_time(1.0 / 6.0))
+    forest = tr.Forest.build_pch(3, np.geomspace(3e10, 1e11, 3), t0, cosmo, tr.pch_params(5e9, t5), seed=4)
+    wp = tr.walk_params(t0, merging_multiplier=0.2)
+    sched = tr.Schedule(forest, cosmo, wp)
+    y0 = forest.initial_state(cosmo, wp)
+    h0 = -np.ones(forest.n_nodes)
+    flags = gp.FLAG_REUSE_STEP_SIZE
+
+    def evolve_one(col, npar, a, b, h):
+        yo, ho, _, st, _ = oracle.evolve(col[:, None], np.asarray(npar)[:, None], [a], [b], [h], physics,
+                                         cooling_table, 1e-6, 1e-3, flags)
+        return yo[:, 0], ho[0], st[0]
+
+    log = [(int(o["kind"]), int(o["a"]), int(o["b"]), float(o["t0"]), float(o["t1"]), list(o["node_par"]))
+           for o in sched.ops]
+    yp, hp, stp = ot.run_sequential(log, y0, h0, evolve_one, lambda y, a, b: ot.merge_galaxy_simple(y, a, b, 0.25))
+    yc, hc, stc, bad = oracle.tree_run(sched.ops, y0, h0, forest.tree_of_node(), forest.n_trees, physics,
+                                       cooling_table, 1e-6, 1e-3, flags, 0.25, nthreads=3)
+    assert bad == 0
+    assert np.array_equal(yc, yp) and np.array_equal(hc, hp) and np.array_equal(stc, stp)
