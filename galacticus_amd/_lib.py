@@ -72,7 +72,8 @@ def load():
 
 SURROGATE_EXPORTS = [
     "gal_mlp_param_count", "gal_mlp_create", "gal_mlp_destroy", "gal_mlp_set_params", "gal_mlp_get_params",
-    "gal_mlp_set_adam", "gal_mlp_set_loss", "gal_mlp_forward_d", "gal_mlp_gradient_d", "gal_mlp_set_dataset",
+    "gal_mlp_set_adam", "gal_mlp_set_loss", "gal_mlp_forward_d", "gal_mlp_forward_layers_d", "gal_mlp_gradient_d",
+    "gal_mlp_set_dataset",
     "gal_mlp_train_d", "gal_surrogate_encode_d", "gal_surrogate_apply_d", "gal_surrogate_evolve_d",
 ]
 
@@ -95,6 +96,7 @@ def _load_surrogate(L):
     L.gal_mlp_set_adam.argtypes = [vp, c_double, c_double, c_double, c_double, c_double, i64]
     L.gal_mlp_set_loss.argtypes = [vp, fp, fp, ip, c_double]
     L.gal_mlp_forward_d.argtypes = [vp, i64, vp, vp, vp]
+    L.gal_mlp_forward_layers_d.argtypes = [vp, i64, vp, vp, vp]
     L.gal_mlp_gradient_d.argtypes = [vp, c_int, vp, vp, vp, vp, vp]
     L.gal_mlp_set_dataset.argtypes = [vp, i64, vp, vp, vp]
     L.gal_mlp_train_d.argtypes = [vp, c_int, c_int, vp, vp]

@@ -166,6 +166,147 @@ __global__ __launch_bounds__(256) void gemm_kernel(GemmArgs g) {
       }
 }
 
+// ---- fused inference forward ---------------------------------------------
+// The whole network for 128 rows per workgroup in one kernel: the rows'
+// activations stay in LDS from layer to layer, and each layer's transposed
+// bf16 weights ([n][k], k contiguous) stream through LDS in 32-wide k chunks,
+// the next chunk's global loads in flight while the current chunk's MFMAs
+// run.  Wave w computes all 128 rows x columns [64w, 64w+64) of a hidden
+// layer (8 x 4 tiles of 16 x 16, 32 MFMAs per chunk); for the 32-column output
+// layer waves 0 and 1 take one column tile each.  The arithmetic is the
+// per-layer GEMMs' (same MFMA, same k order, same bias + SiLU epilogue and
+// bf16 rounding), so the outputs are bit-identical to them.
+constexpr int FR = 128;                 // rows per workgroup
+constexpr int LDH = kHid + 8;           // activation row stride (bf16)
+constexpr int LDX = kIn + 8;            // input row stride (bf16)
+constexpr int LDW = TK + 8;             // weight chunk row stride (bf16)
+constexpr size_t kFusedLds = sizeof(uint16_t) * ((size_t)FR * LDH + (size_t)FR * LDX + 2 * (size_t)kHid * LDW);
+
+struct FusedArgs {
+  const uint16_t* x;                 // [n][kIn] bf16
+  const uint16_t* wt[kLayers];       // [kLdW[l]][kFanIn[l]] bf16
+  const float* bias[kLayers];
+  float* out;                        // [n][kOutLd]
+  int n;
+};
+
+template <int K, int NCOLS, bool LAST>
+__device__ __forceinline__ void fused_layer(const FusedArgs& a, int l, const uint16_t* __restrict__ A, int lda,
+                                            uint16_t* __restrict__ H, uint16_t* __restrict__ W, int r0) {
+  constexpr int NK = K / TK;
+  constexpr int JT = LAST ? 1 : 4;                       // column tiles per wave
+  constexpr int CHUNK_VEC = NCOLS * TK / 8;              // uint4 per weight chunk
+  constexpr int PER_THREAD = (CHUNK_VEC + 255) / 256;
+  const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+  const bool active = LAST ? wave < 2 : true;
+  const int nbase = LAST ? wave * 16 : wave * 64;
+  const uint16_t* __restrict__ wt = a.wt[l];
+  const uint4 zero = make_uint4(0u, 0u, 0u, 0u);
+  uint4 pre[PER_THREAD];
+  auto fetch = [&](int kc) {
+#pragma unroll
+    for (int e = 0; e < PER_THREAD; ++e) {
+      const int v = tid + e * 256;
+      pre[e] = zero;
+      if (v < CHUNK_VEC) {
+        const int nrow = v / (TK / 8), c = (v % (TK / 8)) * 8;
+        pre[e] = *reinterpret_cast<const uint4*>(wt + (size_t)nrow * K + kc * TK + c);
+      }
+    }
+  };
+  auto stash = [&](int buf) {
+#pragma unroll
+    for (int e = 0; e < PER_THREAD; ++e) {
+      const int v = tid + e * 256;
+      if (v < CHUNK_VEC) {
+        const int nrow = v / (TK / 8), c = (v % (TK / 8)) * 8;
+        *reinterpret_cast<uint4*>(W + (size_t)buf * kHid * LDW + nrow * LDW + c) = pre[e];
+      }
+    }
+  };
+  f32x4 acc[8][JT];
+#pragma unroll
+  for (int i = 0; i < 8; ++i)
+#pragma unroll
+    for (int j = 0; j < JT; ++j) acc[i][j] = f32x4{0.0f, 0.0f, 0.0f, 0.0f};
+  fetch(0);
+  stash(0);
+  __syncthreads();
+  for (int kc = 0; kc < NK; ++kc) {
+    const int buf = kc & 1;
+    if (kc + 1 < NK) fetch(kc + 1);
+    if (active) {
+      bf16x8 b[JT];
+#pragma unroll
+      for (int j = 0; j < JT; ++j) {
+        const uint4 v = *reinterpret_cast<const uint4*>(W + (size_t)buf * kHid * LDW +
+                                                        (nbase + j * 16 + (lane & 15)) * LDW + (lane >> 4) * 8);
+        b[j] = __builtin_bit_cast(bf16x8, v);
+      }
+#pragma unroll
+      for (int i = 0; i < 8; ++i) {
+        const uint4 v = *reinterpret_cast<const uint4*>(A + (i * 16 + (lane & 15)) * lda + kc * TK + (lane >> 4) * 8);
+        const bf16x8 av = __builtin_bit_cast(bf16x8, v);
+#pragma unroll
+        for (int j = 0; j < JT; ++j) acc[i][j] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(av, b[j], acc[i][j], 0, 0, 0);
+      }
+    }
+    if (kc + 1 < NK) stash(buf ^ 1);
+    __syncthreads();
+  }
+  // every wave has read this layer's input: the epilogue may overwrite H
+  if (!active) return;
+  const float* __restrict__ bias = a.bias[l];
+#pragma unroll
+  for (int i = 0; i < 8; ++i)
+#pragma unroll
+    for (int j = 0; j < JT; ++j)
+#pragma unroll
+      for (int r = 0; r < 4; ++r) {
+        const int row = i * 16 + (lane >> 4) * 4 + r;
+        const int col = nbase + j * 16 + (lane & 15);
+        const float z = acc[i][j][r] + bias[LAST ? (col < kOut ? col : 0) : col];
+        if (LAST) {
+          if (col < kOut && r0 + row < a.n) a.out[(size_t)(r0 + row) * kOutLd + col] = z;
+        } else {
+          H[row * LDH + col] = f2bf(silu(z));
+        }
+      }
+}
+
+__global__ __launch_bounds__(256, 1) void mlp_forward_fused(FusedArgs a) {
+  extern __shared__ __attribute__((aligned(16))) uint16_t fused_lds[];
+  uint16_t* H = fused_lds;
+  uint16_t* X = H + FR * LDH;
+  uint16_t* W = X + FR * LDX;
+  const int tid = threadIdx.x;
+  const int r0 = blockIdx.x * FR;
+  const uint4 zero = make_uint4(0u, 0u, 0u, 0u);
+  for (int e = tid; e < FR * (kIn / 8); e += 256) {
+    const int r = e / (kIn / 8), c = (e % (kIn / 8)) * 8;
+    uint4 v = zero;
+    if (r0 + r < a.n) v = *reinterpret_cast<const uint4*>(a.x + (size_t)(r0 + r) * kIn + c);
+    *reinterpret_cast<uint4*>(X + r * LDX + c) = v;
+  }
+  __syncthreads();
+  fused_layer<kIn, kHid, false>(a, 0, X, LDX, H, W, r0);
+  __syncthreads();
+  fused_layer<kHid, kHid, false>(a, 1, H, LDH, H, W, r0);
+  __syncthreads();
+  fused_layer<kHid, kHid, false>(a, 2, H, LDH, H, W, r0);
+  __syncthreads();
+  fused_layer<kHid, kOutLd, true>(a, 3, H, LDH, H, W, r0);
+}
+
+// Transposed bf16 copy of one layer's weights: wt[n][k] = wbf[k][n].
+__global__ __launch_bounds__(256) void transpose_weights_kernel(const uint16_t* __restrict__ wbf, int K, int N,
+                                                                uint16_t* __restrict__ wt) {
+  const int e = blockIdx.x * blockDim.x + threadIdx.x;
+  if (e >= K * N) return;
+  const int k = e / N, n = e % N;
+  wt[(size_t)n * K + k] = wbf[e];
+}
+
 // ---- training support kernels ----------------------------------------------
 // Network outputs enter sinh(); beyond +-20 they are clipped (sinh(20) = 2.4e8).
 constexpr float kOutClip = 20.0f;
@@ -216,16 +357,25 @@ __global__ __launch_bounds__(256) void loss_kernel(LossArgs a) {
   if (threadIdx.x == 0) a.loss_partial[blockIdx.x] = red[0];
 }
 
-// Column sums of a bf16 gradient, one fp32 partial per batch split.
+// Column sums of a bf16 gradient, one fp32 partial per batch split.  A
+// workgroup covers 64 columns of one split: each wave reads whole 128-byte
+// row segments (64 consecutive columns) of every fourth row, and the four
+// waves' sums are added in LDS in a fixed order (deterministic).
 __global__ __launch_bounds__(256) void colsum_kernel(const uint16_t* __restrict__ dz, int B, int N, int ld,
                                                      int rows_per_split, float* __restrict__ partial) {
-  const int n = blockIdx.x * blockDim.x + threadIdx.x;
+  __shared__ float red[4][64];
+  const int col = threadIdx.x & 63, q = threadIdx.x >> 6;
+  const int n = blockIdx.x * 64 + col;
   const int s = blockIdx.y;
-  if (n >= N) return;
   const int r0 = s * rows_per_split, r1 = min(B, r0 + rows_per_split);
   float acc = 0.0f;
-  for (int r = r0; r < r1; ++r) acc += bf2f(dz[(size_t)r * ld + n]);
-  partial[(size_t)s * ld + n] = acc;
+  if (n < N) {
+#pragma unroll 8
+    for (int r = r0 + q; r < r1; r += 4) acc += bf2f(dz[(size_t)r * ld + n]);
+  }
+  red[q][col] = acc;
+  __syncthreads();
+  if (q == 0 && n < N) partial[(size_t)s * ld + n] = (red[0][col] + red[1][col]) + (red[2][col] + red[3][col]);
 }
 
 struct AdamArgs {
@@ -627,6 +777,7 @@ struct gal_mlp {
   float* d_m = nullptr;
   float* d_v = nullptr;
   uint16_t* d_wbf[kLayers] = {};
+  uint16_t* d_wt[kLayers] = {};   // transposed bf16 weights for the fused inference kernel
   uint16_t* d_x = nullptr;
   float* d_t = nullptr;
   uint16_t* d_z[3] = {};
@@ -730,7 +881,7 @@ int backward(gal_mlp* m, int n, const uint16_t* x, hipStream_t s) {
     gw.k_split = rps;
     gw.out_f = m->d_pw + m->pw_off[l];
     GAL_TRY((gemm<true, false, EPI_PARTIAL>(gw, splits, s)));
-    hipLaunchKernelGGL(colsum_kernel, dim3((kLdW[l] + 255) / 256, splits), dim3(256), 0, s, grad, n, kLdW[l], ld_grad,
+    hipLaunchKernelGGL(colsum_kernel, dim3((kLdW[l] + 63) / 64, splits), dim3(256), 0, s, grad, n, kLdW[l], ld_grad,
                        rps, m->d_pb + m->pb_off[l]);
     if (l == 0) break;
     // gradient wrt the pre-activation of layer l-1: (grad . W_l^T) * silu'(z_{l-1})
@@ -819,9 +970,39 @@ int ensure_inference(gal_mlp* m, int rows) {
   return GAL_OK;
 }
 
-int infer(gal_mlp* m, int n, const uint16_t* x, float* out, hipStream_t s) {
+// Per-layer GEMM path (the training kernels' forward).
+int infer_layers(gal_mlp* m, int n, const uint16_t* x, float* out, hipStream_t s) {
   uint16_t* h[3] = {m->d_ih[0], m->d_ih[1], m->d_ih[0]};
   return forward(m, n, x, nullptr, h, out, s);
+}
+
+// Refresh the transposed weights from the current bf16 weights.
+int prepare_fused(gal_mlp* m, hipStream_t s) {
+  static bool attr_set = false;
+  if (!attr_set) {
+    static_assert(kFusedLds <= 163840, "fused forward LDS must fit a CU");
+    HIP_TRY(hipFuncSetAttribute((const void*)mlp_forward_fused, hipFuncAttributeMaxDynamicSharedMemorySize,
+                                (int)kFusedLds));
+    attr_set = true;
+  }
+  for (int l = 0; l < kLayers; ++l) {
+    const int e = kFanIn[l] * kLdW[l];
+    hipLaunchKernelGGL(transpose_weights_kernel, dim3((e + 255) / 256), dim3(256), 0, s, m->d_wbf[l], kFanIn[l],
+                       kLdW[l], m->d_wt[l]);
+  }
+  return hipGetLastError() == hipSuccess ? GAL_OK : GAL_E_HIP;
+}
+
+// The fused one-kernel forward (after prepare_fused).
+int infer(gal_mlp* m, int n, const uint16_t* x, float* out, hipStream_t s) {
+  if (n <= 0) return GAL_OK;
+  FusedArgs a{};
+  a.x = x;
+  for (int l = 0; l < kLayers; ++l) { a.wt[l] = m->d_wt[l]; a.bias[l] = m->d_param + m->b_off[l]; }
+  a.out = out;
+  a.n = n;
+  hipLaunchKernelGGL(mlp_forward_fused, dim3((n + FR - 1) / FR), dim3(256), kFusedLds, s, a);
+  return hipGetLastError() == hipSuccess ? GAL_OK : GAL_E_HIP;
 }
 
 }  // namespace
@@ -863,7 +1044,8 @@ int gal_mlp_create(int batch, gal_mlp** out) {
             hipMalloc(&m->d_cursor, sizeof(long long)) == hipSuccess &&
             hipMalloc(&m->d_local, sizeof(int)) == hipSuccess;
   for (int l = 0; l < kLayers && ok; ++l)
-    ok = hipMalloc(&m->d_wbf[l], sizeof(uint16_t) * kFanIn[l] * kLdW[l]) == hipSuccess;
+    ok = hipMalloc(&m->d_wbf[l], sizeof(uint16_t) * kFanIn[l] * kLdW[l]) == hipSuccess &&
+         hipMalloc(&m->d_wt[l], sizeof(uint16_t) * kFanIn[l] * kLdW[l]) == hipSuccess;
   for (int l = 0; l < 3 && ok; ++l)
     ok = hipMalloc(&m->d_z[l], sizeof(uint16_t) * kHid * B) == hipSuccess &&
          hipMalloc(&m->d_h[l], sizeof(uint16_t) * kHid * B) == hipSuccess;
@@ -895,7 +1077,7 @@ int gal_mlp_destroy(gal_mlp* m) {
   if (!m) return GAL_OK;
   release_graph(m);
   hipFree(m->d_param); hipFree(m->d_m); hipFree(m->d_v);
-  for (int l = 0; l < kLayers; ++l) hipFree(m->d_wbf[l]);
+  for (int l = 0; l < kLayers; ++l) { hipFree(m->d_wbf[l]); hipFree(m->d_wt[l]); }
   hipFree(m->d_x); hipFree(m->d_t);
   for (int l = 0; l < 3; ++l) { hipFree(m->d_z[l]); hipFree(m->d_h[l]); }
   hipFree(m->d_out); hipFree(m->d_dout); hipFree(m->d_dz[0]); hipFree(m->d_dz[1]);
@@ -969,10 +1151,23 @@ int gal_mlp_forward_d(gal_mlp* m, int64_t n, const uint16_t* x, float* out, void
   hipStream_t s = (hipStream_t)stream;
   const int chunk = (int)std::min<int64_t>(n, kInferChunk);
   if (n == 0) return GAL_OK;
-  GAL_TRY(ensure_inference(m, chunk));
+  GAL_TRY(prepare_fused(m, s));
   for (int64_t c0 = 0; c0 < n; c0 += chunk) {
     const int nc = (int)std::min<int64_t>(chunk, n - c0);
     GAL_TRY(infer(m, nc, x + c0 * kIn, out + c0 * kOutLd, s));
+  }
+  return GAL_OK;
+}
+
+int gal_mlp_forward_layers_d(gal_mlp* m, int64_t n, const uint16_t* x, float* out, void* stream) {
+  if (!m || n < 0 || (n > 0 && (!x || !out))) return GAL_E_INVALID;
+  hipStream_t s = (hipStream_t)stream;
+  const int chunk = (int)std::min<int64_t>(n, kInferChunk);
+  if (n == 0) return GAL_OK;
+  GAL_TRY(ensure_inference(m, chunk));
+  for (int64_t c0 = 0; c0 < n; c0 += chunk) {
+    const int nc = (int)std::min<int64_t>(chunk, n - c0);
+    GAL_TRY(infer_layers(m, nc, x + c0 * kIn, out + c0 * kOutLd, s));
   }
   return GAL_OK;
 }
@@ -1105,6 +1300,7 @@ int gal_surrogate_evolve_d(gal_ode_batch* h, gal_mlp* m, const float* norm, cons
   if (!m->d_rej_count) HIP_TRY(hipMalloc(&m->d_rej_count, sizeof(int)));
   const int chunk = std::min(n, kInferChunk);
   GAL_TRY(ensure_inference(m, chunk));
+  GAL_TRY(prepare_fused(m, s));
   HIP_TRY(hipMemsetAsync(m->d_rej_count, 0, sizeof(int), s));
   for (int c0 = 0; c0 < n; c0 += chunk) {
     const int nc = std::min(chunk, n - c0);

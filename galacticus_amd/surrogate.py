@@ -125,13 +125,16 @@ class SurrogateMLP:
                                            hv.ctypes.data_as(ctypes.POINTER(ctypes.c_int)), head_weight),
                    "gal_mlp_set_loss")
 
-    def forward(self, x, out=None, stream=None):
+    def forward(self, x, out=None, stream=None, layers=False):
+        """Network outputs (n x 32 fp32; column 31 stays as allocated, zero).
+        layers=True runs the per-layer GEMM path instead of the fused kernel."""
         import torch
         n = int(x.shape[0])
         if out is None:
-            out = torch.empty((n, OUT_LD), dtype=torch.float32, device=x.device)
-        _lib.check(self.L.gal_mlp_forward_d(self.handle, n, _ptr(x), _ptr(out), _stream(stream)),
-                   "gal_mlp_forward_d")
+            out = torch.zeros((n, OUT_LD), dtype=torch.float32, device=x.device)
+        fn = self.L.gal_mlp_forward_layers_d if layers else self.L.gal_mlp_forward_d
+        _lib.check(fn(self.handle, n, _ptr(x), _ptr(out), _stream(stream)),
+                   "gal_mlp_forward_layers_d" if layers else "gal_mlp_forward_d")
         return out
 
     def gradient(self, x, target, stream=None):

@@ -93,15 +93,23 @@ int gal_mlp_set_adam(gal_mlp* m, double lr, double lr_min, double beta1, double 
                      int64_t total_steps);
 
 /* Loss per row: sum_v w_v r_v^2 / sum_v w_v + head_weight (o_30 - ln(e + 1e-7))^2
-   with r_v = o_v - target_v (standardised) for v < 30 and
-   e = max over head_vars of |r_v| * target_sigma_v (the step's error in
-   asinh units), raised to at least 1 when target column 31 is nonzero.
+   with r_v = o_v - target_v for v < 30 (targets are asinh((r - m)/sigma) of
+   the residuals) and e = max over head_vars of
+   target_sigma_v |sinh(o_v) - sinh(target_v)| (the step's error in asinh
+   units of the state; o_v clipped to +-20), raised to at least 1 when target
+   column 31 is nonzero.
    var_weight, target_sigma, head_vars: host arrays of 30. */
 int gal_mlp_set_loss(gal_mlp* m, const float* var_weight, const float* target_sigma, const int* head_vars,
                      double head_weight);
 
-/* Inference: x (device bf16, n x 32) -> out (device fp32, n x 32). */
+/* Inference: x (device bf16, n x 32) -> out (device fp32, n x 32; column 31
+   is not written).  One fused kernel runs all four layers for 128 rows per
+   workgroup, activations kept in LDS. */
 int gal_mlp_forward_d(gal_mlp* m, int64_t n, const uint16_t* x, float* out, void* stream);
+
+/* The same forward as four per-layer GEMM launches through global memory
+   (the training step's forward); bit-identical to gal_mlp_forward_d. */
+int gal_mlp_forward_layers_d(gal_mlp* m, int64_t n, const uint16_t* x, float* out, void* stream);
 
 /* One forward + backward pass on n <= batch rows without an update:
    grad (device fp32, param_count) and loss (device fp32, 1; mean over rows). */

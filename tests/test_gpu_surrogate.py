@@ -1,6 +1,7 @@
 """GPU tests of the learned surrogate (include/gal_surrogate.h) through the C ABI.
 
 Bars:
+  * the fused inference forward bit-identical to the per-layer GEMM forward;
   * network forward and gradients against the CPU restatement with the same
     bf16 rounding points (oracle/mlp.py): outputs within 2e-2 of their scale,
     gradients within 3e-2 in norm per layer (the two differ by fp32 summation
@@ -79,6 +80,23 @@ def test_forward_matches_oracle(torch_cuda):
     print("forward max err", err.max(), "scale", np.abs(ref).max(), flush=True)
     assert err.max() <= 2e-2 * np.abs(ref).max()
     assert np.median(err) <= 1e-3 * np.abs(ref).max()
+
+
+def test_fused_forward_equals_layer_path(torch_cuda):
+    """The fused one-kernel forward and the four per-layer GEMM launches run
+    the same MFMA sums and epilogues: bit-identical outputs, also for a row
+    count that is not a multiple of the 128-row workgroup tile and across
+    several inference chunks."""
+    torch = torch_cuda
+    m, p = _net()
+    rng = np.random.default_rng(4)
+    for n in (1, 300, 131072 + 77):
+        x = _bits(torch, om.bf16(rng.normal(0, 1, (n, 32)).astype(np.float32)))
+        fused = m.forward(x)
+        layers = m.forward(x, layers=True)
+        torch.cuda.synchronize()
+        assert torch.equal(fused[:, :31], layers[:, :31]), n
+        assert bool(torch.isfinite(fused[:, :31]).all())
 
 
 def test_gradient_matches_oracle(torch_cuda):
