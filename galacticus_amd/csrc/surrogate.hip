@@ -168,19 +168,19 @@ __global__ __launch_bounds__(256) void gemm_kernel(GemmArgs g) {
 
 // ---- fused inference forward ---------------------------------------------
 // The whole network for 128 rows per workgroup in one kernel: the rows'
-// activations stay in LDS from layer to layer, and each layer's transposed
-// bf16 weights ([n][k], k contiguous) stream through LDS in 32-wide k chunks,
-// the next chunk's global loads in flight while the current chunk's MFMAs
-// run.  Wave w computes all 128 rows x columns [64w, 64w+64) of a hidden
-// layer (8 x 4 tiles of 16 x 16, 32 MFMAs per chunk); for the 32-column output
-// layer waves 0 and 1 take one column tile each.  The arithmetic is the
+// activations stay in LDS from layer to layer (76 KiB, so two workgroups
+// share a CU).  Wave w computes all 128 rows x columns [64w, 64w+64) of a
+// hidden layer (8 x 4 tiles of 16 x 16); for the 32-column output layer waves
+// 0 and 1 take one column tile each.  A wave's columns are its own, so it
+// reads its weight fragments straight from the transposed bf16 weights
+// ([n][k], k contiguous) into registers, four k chunks at a time, with no LDS
+// staging and no workgroup barrier inside a layer.  The arithmetic is the
 // per-layer GEMMs' (same MFMA, same k order, same bias + SiLU epilogue and
 // bf16 rounding), so the outputs are bit-identical to them.
 constexpr int FR = 128;                 // rows per workgroup
 constexpr int LDH = kHid + 8;           // activation row stride (bf16)
 constexpr int LDX = kIn + 8;            // input row stride (bf16)
-constexpr int LDW = TK + 8;             // weight chunk row stride (bf16)
-constexpr size_t kFusedLds = sizeof(uint16_t) * ((size_t)FR * LDH + (size_t)FR * LDX + 2 * (size_t)kHid * LDW);
+constexpr size_t kFusedLds = sizeof(uint16_t) * ((size_t)FR * LDH + (size_t)FR * LDX);
 
 struct FusedArgs {
   const uint16_t* x;                 // [n][kIn] bf16
@@ -190,71 +190,47 @@ struct FusedArgs {
   int n;
 };
 
-template <int K, int NCOLS, bool LAST>
+template <int K, bool LAST>
 __device__ __forceinline__ void fused_layer(const FusedArgs& a, int l, const uint16_t* __restrict__ A, int lda,
-                                            uint16_t* __restrict__ H, uint16_t* __restrict__ W, int r0) {
+                                            uint16_t* __restrict__ H, int r0) {
   constexpr int NK = K / TK;
+  constexpr int KG = NK < 4 ? NK : 4;                    // k chunks per register group
   constexpr int JT = LAST ? 1 : 4;                       // column tiles per wave
-  constexpr int CHUNK_VEC = NCOLS * TK / 8;              // uint4 per weight chunk
-  constexpr int PER_THREAD = (CHUNK_VEC + 255) / 256;
-  const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
   const bool active = LAST ? wave < 2 : true;
   const int nbase = LAST ? wave * 16 : wave * 64;
   const uint16_t* __restrict__ wt = a.wt[l];
-  const uint4 zero = make_uint4(0u, 0u, 0u, 0u);
-  uint4 pre[PER_THREAD];
-  auto fetch = [&](int kc) {
-#pragma unroll
-    for (int e = 0; e < PER_THREAD; ++e) {
-      const int v = tid + e * 256;
-      pre[e] = zero;
-      if (v < CHUNK_VEC) {
-        const int nrow = v / (TK / 8), c = (v % (TK / 8)) * 8;
-        pre[e] = *reinterpret_cast<const uint4*>(wt + (size_t)nrow * K + kc * TK + c);
-      }
-    }
-  };
-  auto stash = [&](int buf) {
-#pragma unroll
-    for (int e = 0; e < PER_THREAD; ++e) {
-      const int v = tid + e * 256;
-      if (v < CHUNK_VEC) {
-        const int nrow = v / (TK / 8), c = (v % (TK / 8)) * 8;
-        *reinterpret_cast<uint4*>(W + (size_t)buf * kHid * LDW + nrow * LDW + c) = pre[e];
-      }
-    }
-  };
   f32x4 acc[8][JT];
 #pragma unroll
   for (int i = 0; i < 8; ++i)
 #pragma unroll
     for (int j = 0; j < JT; ++j) acc[i][j] = f32x4{0.0f, 0.0f, 0.0f, 0.0f};
-  fetch(0);
-  stash(0);
-  __syncthreads();
-  for (int kc = 0; kc < NK; ++kc) {
-    const int buf = kc & 1;
-    if (kc + 1 < NK) fetch(kc + 1);
-    if (active) {
-      bf16x8 b[JT];
+  if (active) {
+    for (int g0 = 0; g0 < NK; g0 += KG) {
+      uint4 bq[KG][JT];
 #pragma unroll
-      for (int j = 0; j < JT; ++j) {
-        const uint4 v = *reinterpret_cast<const uint4*>(W + (size_t)buf * kHid * LDW +
-                                                        (nbase + j * 16 + (lane & 15)) * LDW + (lane >> 4) * 8);
-        b[j] = __builtin_bit_cast(bf16x8, v);
-      }
+      for (int kk = 0; kk < KG; ++kk)
 #pragma unroll
-      for (int i = 0; i < 8; ++i) {
-        const uint4 v = *reinterpret_cast<const uint4*>(A + (i * 16 + (lane & 15)) * lda + kc * TK + (lane >> 4) * 8);
-        const bf16x8 av = __builtin_bit_cast(bf16x8, v);
+        for (int j = 0; j < JT; ++j)
+          bq[kk][j] = *reinterpret_cast<const uint4*>(wt + (size_t)(nbase + j * 16 + (lane & 15)) * K +
+                                                      (g0 + kk) * TK + (lane >> 4) * 8);
 #pragma unroll
-        for (int j = 0; j < JT; ++j) acc[i][j] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(av, b[j], acc[i][j], 0, 0, 0);
+      for (int kk = 0; kk < KG; ++kk) {
+#pragma unroll
+        for (int i = 0; i < 8; ++i) {
+          const uint4 v = *reinterpret_cast<const uint4*>(A + (i * 16 + (lane & 15)) * lda + (g0 + kk) * TK +
+                                                          (lane >> 4) * 8);
+          const bf16x8 av = __builtin_bit_cast(bf16x8, v);
+#pragma unroll
+          for (int j = 0; j < JT; ++j)
+            acc[i][j] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(av, __builtin_bit_cast(bf16x8, bq[kk][j]), acc[i][j],
+                                                                0, 0, 0);
+        }
       }
     }
-    if (kc + 1 < NK) stash(buf ^ 1);
-    __syncthreads();
   }
-  // every wave has read this layer's input: the epilogue may overwrite H
+  // every wave has read this layer's input before any epilogue overwrites H
+  __syncthreads();
   if (!active) return;
   const float* __restrict__ bias = a.bias[l];
 #pragma unroll
@@ -274,11 +250,10 @@ __device__ __forceinline__ void fused_layer(const FusedArgs& a, int l, const uin
       }
 }
 
-__global__ __launch_bounds__(256, 1) void mlp_forward_fused(FusedArgs a) {
+__global__ __launch_bounds__(256, 2) void mlp_forward_fused(FusedArgs a) {
   extern __shared__ __attribute__((aligned(16))) uint16_t fused_lds[];
   uint16_t* H = fused_lds;
   uint16_t* X = H + FR * LDH;
-  uint16_t* W = X + FR * LDX;
   const int tid = threadIdx.x;
   const int r0 = blockIdx.x * FR;
   const uint4 zero = make_uint4(0u, 0u, 0u, 0u);
@@ -289,13 +264,13 @@ __global__ __launch_bounds__(256, 1) void mlp_forward_fused(FusedArgs a) {
     *reinterpret_cast<uint4*>(X + r * LDX + c) = v;
   }
   __syncthreads();
-  fused_layer<kIn, kHid, false>(a, 0, X, LDX, H, W, r0);
+  fused_layer<kIn, false>(a, 0, X, LDX, H, r0);
   __syncthreads();
-  fused_layer<kHid, kHid, false>(a, 1, H, LDH, H, W, r0);
+  fused_layer<kHid, false>(a, 1, H, LDH, H, r0);
   __syncthreads();
-  fused_layer<kHid, kHid, false>(a, 2, H, LDH, H, W, r0);
+  fused_layer<kHid, false>(a, 2, H, LDH, H, r0);
   __syncthreads();
-  fused_layer<kHid, kOutLd, true>(a, 3, H, LDH, H, W, r0);
+  fused_layer<kHid, true>(a, 3, H, LDH, H, r0);
 }
 
 // Transposed bf16 copy of one layer's weights: wt[n][k] = wbf[k][n].
