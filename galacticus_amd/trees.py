@@ -233,6 +233,30 @@ class Forest:
     def roots(self):
         return self.tree_offset[:-1]
 
+    def split(self, n_parts: int):
+        """Split into n_parts forests of consecutive trees (balanced by node
+        count).  Returns [(forest, first_node)]: node i of part k is node
+        first_node + i of this forest.  Trees are independent, so evolving the
+        parts separately applies exactly the same operations to every node."""
+        n_parts = max(1, min(int(n_parts), self.n_trees))
+        sizes = np.diff(self.tree_offset)
+        cum = np.cumsum(sizes)
+        cuts = [0]
+        for k in range(1, n_parts):
+            cuts.append(int(np.searchsorted(cum, k * cum[-1] / n_parts)) + 1)
+        cuts.append(self.n_trees)
+        cuts = sorted(set(min(max(c, 0), self.n_trees) for c in cuts))
+        parts = []
+        for a, b in zip(cuts[:-1], cuts[1:]):
+            if b <= a:
+                continue
+            n0, n1 = int(self.tree_offset[a]), int(self.tree_offset[b])
+            rel = lambda v: np.where(v >= 0, v - n0, -1).astype(np.int32)
+            parts.append((Forest.from_arrays(self.tree_offset[a:b + 1] - n0, rel(self.parent[n0:n1]),
+                                             rel(self.first_child[n0:n1]), rel(self.sibling[n0:n1]),
+                                             self.mass[n0:n1], self.time[n0:n1]), n0))
+        return parts
+
     def tree_of_node(self):
         """Index of the tree that holds each node (nodes of a tree are
         contiguous, tree_offset[t] <= node < tree_offset[t+1])."""
@@ -287,6 +311,30 @@ class Schedule:
             self.close()
         except Exception:
             pass
+
+
+def evolve_forests(evolvers, schedules, ys, streams, major_ratio=0.25):
+    """Evolve independent forests concurrently: forest k on evolver k and
+    stream k, each driven by its own host thread (the library call releases
+    the interpreter lock), so one forest's waves overlap the others'.
+    Returns the per-node status tensors."""
+    import threading
+    out = [None] * len(schedules)
+    err = []
+
+    def run(k):
+        try:
+            out[k] = evolve_forest(evolvers[k], schedules[k], ys[k], major_ratio=major_ratio, stream=streams[k])
+        except Exception as e:  # reported after every thread has joined
+            err.append(e)
+    threads = [threading.Thread(target=run, args=(k,)) for k in range(len(schedules))]
+    for t in threads:
+        t.start()
+    for t in threads:
+        t.join()
+    if err:
+        raise err[0]
+    return out
 
 
 def evolve_forest(evolver, schedule: Schedule, y, scale=None, status=None, major_ratio=0.25, stream=None):

@@ -252,3 +252,25 @@ def test_c_tree_runner_equals_sequential_restatement(cosmo, cooling_table):
                                        cooling_table, 1e-6, 1e-3, flags, 0.25, nthreads=3)
     assert bad == 0
     assert np.array_equal(yc, yp) and np.array_equal(hc, hp) and np.array_equal(stc, stp)
+
+
+def test_forest_split_keeps_trees(cosmo):
+    """Splitting a forest into parts of consecutive trees keeps every tree:
+    the parts' schedules are the whole forest's operations, re-indexed."""
+    t0 = float(wl.scale_factor_to_cosmic_time(1.0))
+    t5 = float(wl.scale_factor_to_cosmic_time(1.0 / 6.0))
+    forest = tr.Forest.build_pch(7, np.geomspace(3e10, 3e11, 7), t0, cosmo, tr.pch_params(5e9, t5), seed=8)
+    wp = tr.walk_params(t0)
+    whole = tr.Schedule(forest, cosmo, wp).ops
+    parts = forest.split(3)
+    assert sum(f.n_nodes for f, _ in parts) == forest.n_nodes and sum(f.n_trees for f, _ in parts) == 7
+    ops = []
+    for f, n0 in parts:
+        o = tr.Schedule(f, cosmo, wp).ops.copy()
+        o["a"] += n0
+        o["b"] = np.where(o["b"] >= 0, o["b"] + n0, o["b"])
+        ops.append(o)
+    ops = np.concatenate(ops)
+    key = lambda o: sorted(zip(o["kind"].tolist(), o["a"].tolist(), o["b"].tolist(), o["t0"].tolist(),
+                               o["t1"].tolist()))
+    assert key(ops) == key(whole)
