@@ -1,0 +1,78 @@
+"""Full-size GPU checks through properties that do not depend on batch size.
+
+On the 1e5-node isolated-halo batch (the integrator's benchmark workload,
+FP64 Cash-Karp at rel 1e-3, abs 1e-6):
+  * every node succeeds;
+  * baryons are conserved: the only source is accretion f_b Mdot_halo dt
+    (circumgalactic_medium/accretion.F90 with accretionHalo simple) and the
+    only sink is the energy radiated and carried off by the black-hole jet,
+    a fraction eff/(1 - eff) <= 0.75 of the black hole's growth (ISCO
+    efficiency at most 0.42 for spin <= 0.998, black_holes/accretion.F90).
+    A Runge-Kutta step preserves linear invariants of the RHS up to rounding,
+    so the residual is held to 1e-7 of the baryon mass (1e-6 for metals, whose
+    flows through the yield can exceed the metal mass itself);
+  * metals are conserved: the only source is the yield of newly formed stars
+    (yield x Delta(stars formed)), the only sink the gas the black hole eats
+    (at most its growth / (1 - eff) of gas, whose metal fraction is below 1);
+  * nodes whose black hole does not grow conserve both to rounding.
+A dropped or doubled flux term of any operator breaks one of these at the
+level of that flux, far above 1e-7.
+"""
+import numpy as np
+import pytest
+
+from galacticus_amd import params as gp
+from galacticus_amd import workloads as wl
+
+pytestmark = pytest.mark.gpu
+
+V = wl.V
+BARYONS = ["HOT_MASS", "HOT_OUTFLOWED_MASS", "HOT_UNACCRETED_MASS", "DISK_MASS_GAS", "DISK_MASS_STELLAR",
+           "SPHEROID_MASS_GAS", "SPHEROID_MASS_STELLAR", "BH_MASS"]
+METALS = ["HOT_ABUNDANCES", "HOT_OUTFLOWED_ABUNDANCES", "HOT_UNACCRETED_ABUNDANCES", "DISK_ABUNDANCES_GAS",
+          "DISK_ABUNDANCES_STELLAR", "SPHEROID_ABUNDANCES_GAS", "SPHEROID_ABUNDANCES_STELLAR"]
+
+
+@pytest.fixture(scope="module")
+def torch_cuda():
+    import torch
+    assert torch.cuda.is_available(), "GPU tests need a GPU"
+    return torch
+
+
+def test_full_batch_conserves_baryons_and_metals(torch_cuda, cooling_table):
+    torch = torch_cuda
+    from galacticus_amd.evolver import NodeEvolverBatched
+    p = gp.physics_params()
+    b = wl.isolated_halos(100000, seed=0, params=p)
+    print("full batch", b.n, "nodes", flush=True)
+    ev = NodeEvolverBatched(b.n, p, cooling_table, odeToleranceAbsolute=1e-6, odeToleranceRelative=1e-3)
+    d = lambda a: torch.from_numpy(np.ascontiguousarray(a)).cuda()
+    y = d(b.y)
+    st = ev.evolve_device(y, d(b.node_par), d(b.t_start), d(b.t_end), d(b.h_step)).cpu().numpy()
+    assert (st == 0).all()
+    y0, y1 = b.y, y.cpu().numpy()
+    fb = p[gp.P["OmegaBaryon"]] / p[gp.P["OmegaMatter"]]
+    yld = p[gp.P["metalYield"]]
+    acc = np.where(b.node_par[4] != 0, 0.0, fb * b.node_par[2] * (b.t_end - b.t_start))
+    mb0 = sum(y0[V[k]] for k in BARYONS)
+    mb1 = sum(y1[V[k]] for k in BARYONS)
+    dbh = y1[V["BH_MASS"]] - y0[V["BH_MASS"]]
+    loss = acc - (mb1 - mb0)
+    scale = mb0 + acc
+    print("baryon loss / scale: min", (loss / scale).min(), "max", (loss / scale).max(), flush=True)
+    assert (loss >= -1e-7 * scale).all()
+    assert (loss <= 0.75 * np.maximum(dbh, 0.0) + 1e-7 * scale).all()
+    formed = ((y1[V["DISK_MASS_STELLAR_FORMED"]] - y0[V["DISK_MASS_STELLAR_FORMED"]]) +
+              (y1[V["SPHEROID_MASS_STELLAR_FORMED"]] - y0[V["SPHEROID_MASS_STELLAR_FORMED"]]))
+    z0 = sum(y0[V[k]] for k in METALS)
+    z1 = sum(y1[V[k]] for k in METALS)
+    zres = (z1 - z0) - yld * formed
+    zscale = z0 + yld * np.abs(formed)
+    print("metal residual / scale: min", (zres / zscale).min(), "max", (zres / zscale).max(), flush=True)
+    assert (zres <= 1e-6 * zscale).all()
+    assert (-zres <= 2.5 * np.maximum(dbh, 0.0) + 1e-6 * zscale).all()
+    quiet = dbh <= 0.0
+    print("nodes without black-hole growth", int(quiet.sum()), flush=True)
+    assert (np.abs(loss[quiet]) <= 1e-7 * scale[quiet]).all()
+    assert (np.abs(zres[quiet]) <= 1e-6 * zscale[quiet]).all()

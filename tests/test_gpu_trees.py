@@ -125,3 +125,30 @@ def test_forest_waves_match_sequential_oracle(torch_cuda, cooling_table):
     mstar_g = yg[V["DISK_MASS_STELLAR"], roots] + yg[V["SPHEROID_MASS_STELLAR"], roots]
     mstar_o = yo[V["DISK_MASS_STELLAR"], roots] + yo[V["SPHEROID_MASS_STELLAR"], roots]
     assert np.allclose(mstar_g, mstar_o, rtol=1e-3)
+
+
+def test_concurrent_forests_equal_one_forest(torch_cuda, cooling_table):
+    """A forest split into independent forests of consecutive trees, evolved
+    concurrently on their own streams and host threads, ends in exactly the
+    state of the whole forest evolved at once (trees never interact)."""
+    torch = torch_cuda
+    from galacticus_amd.evolver import NodeEvolverBatched
+    physics = gp.physics_params()
+    cosmo = tr.cosmology(physics)
+    forest = tr.Forest.build_pch(12, np.geomspace(1e11, 1e12, 12), T0, cosmo, tr.pch_params(2e10, T_Z5), seed=5)
+    wp = tr.walk_params(T0, merging_multiplier=0.2)
+    y0 = forest.initial_state(cosmo, wp)
+    ev = NodeEvolverBatched(forest.n_nodes, physics, cooling_table)
+    yd = torch.from_numpy(y0.copy()).cuda()
+    st = tr.evolve_forest(ev, tr.Schedule(forest, cosmo, wp), yd).cpu().numpy()
+    parts = forest.split(3)
+    print("nodes", forest.n_nodes, "parts", [f.n_nodes for f, _ in parts], flush=True)
+    scheds = [tr.Schedule(f, cosmo, wp) for f, _ in parts]
+    evs = [NodeEvolverBatched(f.n_nodes, physics, cooling_table) for f, _ in parts]
+    ys = [torch.from_numpy(np.ascontiguousarray(y0[:, n0:n0 + f.n_nodes])).cuda() for f, n0 in parts]
+    sts = tr.evolve_forests(evs, scheds, ys, [torch.cuda.Stream() for _ in parts])
+    torch.cuda.synchronize()
+    y_split = np.concatenate([y.cpu().numpy() for y in ys], axis=1)
+    st_split = np.concatenate([s.cpu().numpy() for s in sts])
+    assert np.array_equal(y_split, yd.cpu().numpy())
+    assert np.array_equal(st_split, st)
