@@ -98,26 +98,27 @@ def _timed(fn, reps=2):
 
 
 def surrogate_leg(physics, table, tol_abs, tol_rel, args):
-    """Surrogate training on integrator trajectories, then surrogate+fallback
-    against the plain integrator on held-out node intervals, and a chained
-    evolution of node histories from z=5 to z=0."""
+    """Surrogate training on integrator trajectories of merger trees drawn as
+    the 64-tree workload (roots log-uniform 1e10-1e12 Msun, resolution 1e9
+    Msun, z=5 -> 0; forests of 1024 such trees, seeds 100, 101, ...), then
+    surrogate+fallback against the plain integrator on held-out tree
+    intervals (seeds 200, ...), the 64-tree forest evolved with the surrogate
+    in front of the integrator, and chained isolated-halo histories."""
     import torch
     from galacticus_amd import workloads as wl, surrogate as sg
     from galacticus_amd.evolver import NodeEvolverBatched
-    n_hist = int(args.train_pairs / 12.5) + 1
-    ev = NodeEvolverBatched(max(n_hist, args.eval_pairs), physics, table, odeToleranceAbsolute=tol_abs,
+    ev = NodeEvolverBatched(args.eval_pairs, physics, table, odeToleranceAbsolute=tol_abs,
                             odeToleranceRelative=tol_rel)
     t = time.perf_counter()
-    pairs = sg.generate_pairs(ev, wl.halo_histories(n_hist, seed=0, params=physics), args.train_pairs)
+    pairs = sg.generate_tree_pairs(physics, table, args.train_pairs, seed=100, tol_abs=tol_abs, tol_rel=tol_rel)
     torch.cuda.synchronize()
     gen_s = time.perf_counter() - t
-    log(f"surrogate: {pairs.n} training pairs in {gen_s:.2f} s")
+    log(f"surrogate: {pairs.n} training pairs from tree trajectories in {gen_s:.2f} s")
     mlp, norm, rep = sg.train_surrogate(ev, pairs, epochs=args.train_epochs, batch=8192)
     log(f"surrogate: trained {rep.steps} steps in {rep.seconds:.2f} s ({rep.samples_per_s:.3g} samples/s, "
         f"{rep.tflops:.2f} TFLOP/s), loss {rep.loss_first:.4g} -> {rep.loss_last:.4g}")
     del pairs
-    test = sg.generate_pairs(ev, wl.halo_histories(int(args.eval_pairs / 12.5) + 1, seed=1, params=physics),
-                             args.eval_pairs)
+    test = sg.generate_tree_pairs(physics, table, args.eval_pairs, seed=200, tol_abs=tol_abs, tol_rel=tol_rel)
     n = test.n
     gyr = float((test.t1 - test.t0).sum())
     h0 = -torch.ones(n, dtype=torch.float64, device=test.t0.device)
@@ -153,8 +154,10 @@ def surrogate_leg(physics, table, tol_abs, tol_rel, args):
     ms_ref = y_rk[V["DISK_MASS_STELLAR"]] + y_rk[V["SPHEROID_MASS_STELLAR"]]
     ms_sur = y_s[V["DISK_MASS_STELLAR"]] + y_s[V["SPHEROID_MASS_STELLAR"]]
     ms_err = ((ms_sur - ms_ref).abs() / ms_ref.clamp_min(1e-300))
-    # the same surrogate on chained node histories from z=5 to z=0
+    # the same surrogate on chained node histories from z=5 to z=0, and in
+    # the 64-tree forest
     chain = chained_histories(ev, se, physics, args)
+    forest = surrogate_in_forest(se, physics, table, tol_abs, tol_rel)
     rk_rate, sur_rate = gyr / t_rk, gyr / t_sur
     piped = pipelined_surrogate(ev, mlp, norm, gate, test, h0, launch, physics, table, tol_abs, tol_rel, args)
     log(f"surrogate: RK {rk_rate:.4g} node-Gyr/s, surrogate+fallback {sur_rate:.4g} ({sur_rate / rk_rate:.2f}x), "
@@ -162,9 +165,11 @@ def surrogate_leg(physics, table, tol_abs, tol_rel, args):
     log(f"surrogate pipelined over {piped['streams']} streams: RK {piped['rk_s_per_batch'] * 1e3:.2f} ms/batch, "
         f"surrogate+fallback {piped['surrogate_s_per_batch'] * 1e3:.2f} ms/batch "
         f"({piped['rk_s_per_batch'] / piped['surrogate_s_per_batch']:.2f}x)")
+    args.trained_surrogate = se
     return {
-        "workload": "node-interval steps from node histories (simple timestep shortened by U(0.25,1)), "
-                    "FP64 integrator vs bf16 MLP + gate + FP64 fallback",
+        "workload": "node intervals of merger trees drawn as the 64-tree workload (forests of 1024 trees, roots "
+                    "1e10-1e12 Msun, z=5 -> 0; training seeds 100+, evaluation seeds 200+), FP64 integrator vs "
+                    "bf16 MLP + gate + FP64 fallback",
         "training": {"pairs": int(rep.steps * rep.batch / max(rep.epochs, 1)), "epochs": rep.epochs, "batch": rep.batch,
                      "steps": rep.steps, "seconds": rep.seconds, "samples_per_s": rep.samples_per_s,
                      "data_generation_s": gen_s, "loss_first": rep.loss_first, "loss_last": rep.loss_last,
@@ -177,6 +182,7 @@ def surrogate_leg(physics, table, tol_abs, tol_rel, args):
                       "median_stellar_mass_error_all": float(ms_err.median()),
                       "accepted_relative_error": per_q},
         "chained_to_z0": chain,
+        "forest_64_trees": forest,
         "pipelined": dict(piped, rk_node_gyr_per_s=gyr / piped["rk_s_per_batch"],
                           surrogate_node_gyr_per_s=gyr / piped["surrogate_s_per_batch"],
                           speedup=piped["rk_s_per_batch"] / piped["surrogate_s_per_batch"]),
@@ -245,6 +251,45 @@ def pipelined_surrogate(ev, mlp, norm, gate, test, h0, launch, physics, table, t
     return out
 
 
+def surrogate_in_forest(se, physics, table, tol_abs, tol_rel):
+    """The 64-tree workload evolved by the tree executor with and without the
+    surrogate in front of the integrator: z=0 stellar masses of the roots."""
+    import torch
+    from galacticus_amd import trees as tr, workloads as wl
+    from galacticus_amd.evolver import NodeEvolverBatched
+    from galacticus_amd.surrogate import SurrogateEvolver
+    cosmo = tr.cosmology(physics)
+    t0 = float(wl.scale_factor_to_cosmic_time(1.0))
+    t5 = float(wl.scale_factor_to_cosmic_time(1.0 / 6.0))
+    rng = np.random.Generator(np.random.PCG64(0))
+    forest = tr.Forest.build_pch(64, 10.0 ** rng.uniform(10.0, 12.0, 64), t0, cosmo, tr.pch_params(1e9, t5), seed=0)
+    wp = tr.walk_params(t0)
+    sched = tr.Schedule(forest, cosmo, wp)
+    y0 = forest.initial_state(cosmo, wp)
+    ev = NodeEvolverBatched(forest.n_nodes, physics, table, odeToleranceAbsolute=tol_abs, odeToleranceRelative=tol_rel)
+    se_f = SurrogateEvolver(ev, se.mlp, se.norm, se.gate)
+    out = {}
+    for name, sur in (("rk", None), ("surrogate", se_f)):
+        y = torch.from_numpy(y0).cuda()
+        torch.cuda.synchronize()
+        t = time.perf_counter()
+        tr.evolve_forest(ev, sched, y, surrogate=sur)
+        torch.cuda.synchronize()
+        out[name] = (time.perf_counter() - t, y.cpu().numpy())
+    V = wl.V
+    r = forest.roots()
+    ms = {k: v[1][V["DISK_MASS_STELLAR"], r] + v[1][V["SPHEROID_MASS_STELLAR"], r] for k, v in out.items()}
+    rel = np.abs(ms["surrogate"] - ms["rk"]) / np.maximum(ms["rk"], 1e-300)
+    n_ev = int((sched.ops["kind"] == 0).sum())
+    res = {"roots": int(len(r)), "evolve_intervals": n_ev, "accepted_intervals": se_f.tree_accepted,
+           "median_z0_stellar_mass_error": float(np.median(rel)), "p90_z0_stellar_mass_error": float(np.quantile(rel, 0.9)),
+           "max_z0_stellar_mass_error": float(rel.max()), "rk_seconds": out["rk"][0],
+           "surrogate_seconds": out["surrogate"][0]}
+    log(f"surrogate in the 64-tree forest: {se_f.tree_accepted} of {n_ev} intervals accepted, z=0 M* error median "
+        f"{np.median(rel):.2e}, p90 {np.quantile(rel, 0.9):.2e}; {out['surrogate'][0]:.3f} s vs {out['rk'][0]:.3f} s")
+    return res
+
+
 def chained_histories(ev, se, physics, args, n=20000):
     """Node histories from z=5 to z=0, interval by interval, once with the
     integrator alone and once with the surrogate in front of it."""
@@ -286,7 +331,8 @@ def chained_histories(ev, se, physics, args, n=20000):
             "rk_seconds": times["rk"], "surrogate_seconds": times["surrogate"]}
 
 
-def _forest_run(physics, table, tol_abs, tol_rel, n_trees, log_m, z_start, label, cpu_threads, cpu=True, split=1):
+def _forest_run(physics, table, tol_abs, tol_rel, n_trees, log_m, z_start, label, cpu_threads, cpu=True, split=1,
+                surrogate=None):
     """Build a PCH forest, record its walk, evolve it wave by wave on the GPU
     and (cpu) with the CPU oracle's tree runner over cpu_threads threads;
     compare the z=0 stellar masses of the roots."""
@@ -350,6 +396,31 @@ def _forest_run(physics, table, tol_abs, tol_rel, n_trees, log_m, z_start, label
         if split_s < gpu_s:
             out["gpu_seconds"], out["node_gyr_per_s"] = split_s, gyr / split_s
         del hold, evs
+    y_sur = None
+    if surrogate is not None:
+        # the same forest with the trained surrogate in front of the
+        # integrator for the intervals of non-satellite halos
+        from galacticus_amd.surrogate import SurrogateEvolver
+        ev_t = NodeEvolverBatched(forest.n_nodes, physics, table, odeToleranceAbsolute=tol_abs,
+                                  odeToleranceRelative=tol_rel)
+        se_t = SurrogateEvolver(ev_t, surrogate.mlp, surrogate.norm, surrogate.gate)
+        hold = {}
+
+        def run_sur():
+            y = torch.from_numpy(y0).cuda()
+            hold["st"] = tr.evolve_forest(ev_t, sched, y, surrogate=se_t)
+            hold["y"] = y
+        sur_s, _ = _timed(run_sur)
+        y_sur = hold["y"].cpu().numpy()
+        n_ev = int(k[0])
+        n_ns = int(((sched.ops["kind"] == 0) & (sched.ops["node_par"][:, 4] == 0)).sum())
+        out["surrogate"] = {"gpu_seconds": sur_s, "node_gyr_per_s": gyr / sur_s,
+                            "accepted_intervals": se_t.tree_accepted, "evolve_intervals": n_ev,
+                            "non_satellite_intervals": n_ns, "accepted_fraction": se_t.tree_accepted / max(n_ev, 1),
+                            "statuses_ok": bool((hold["st"].cpu().numpy() == 0).all())}
+        log(f"trees ({label}): with the surrogate {sur_s:.3f} s, {se_t.tree_accepted} of {n_ev} intervals "
+            f"accepted ({se_t.tree_accepted / max(n_ev, 1):.2f})")
+        del hold, ev_t
     if cpu:
         import oracle
         from galacticus_amd import params as gp
@@ -376,6 +447,16 @@ def _forest_run(physics, table, tol_abs, tol_rel, n_trees, log_m, z_start, label
                                          "smf_bins_0.25dex_count_moved": smf_diff, "roots": int(len(r))}
         log(f"trees ({label}): CPU {cpu_s:.2f} s on {cpu_threads} threads, GPU/CPU {cpu_s / out['gpu_seconds']:.1f}x, "
             f"root M* median rel diff {np.median(rel):.2e}, SMF galaxies moved between bins {smf_diff}")
+        if y_sur is not None:
+            ms = y_sur[V["DISK_MASS_STELLAR"], r] + y_sur[V["SPHEROID_MASS_STELLAR"], r]
+            rs = np.abs(ms - mc) / np.maximum(mc, 1e-300)
+            hs = np.histogram(np.log10(np.maximum(ms, 1e-30)), bins)[0]
+            out["surrogate"]["z0_stellar_mass_vs_cpu"] = {
+                "median_relative": float(np.median(rs)), "p90_relative": float(np.quantile(rs, 0.9)),
+                "max_relative": float(rs.max()), "smf_bins_0.25dex_count_moved": int(np.abs(hs - hc).sum()),
+                "roots": int(len(r))}
+            log(f"trees ({label}): surrogate root M* vs CPU median rel {np.median(rs):.2e}, p90 "
+                f"{np.quantile(rs, 0.9):.2e}, SMF galaxies moved {int(np.abs(hs - hc).sum())}")
     return out
 
 
@@ -392,7 +473,7 @@ def trees_leg(physics, table, tol_abs, tol_rel, args):
         out["large"] = _forest_run(physics, table, tol_abs, tol_rel, args.large_trees, (11.0, 13.0), 8.0,
                                    f"{args.large_trees} PCH trees, seed 0, roots log-uniform 1e11-1e13 Msun, "
                                    f"resolution 1e9 Msun, z=8 -> 0", threads, cpu=not args.no_cpu,
-                                   split=args.tree_streams)
+                                   split=args.tree_streams, surrogate=getattr(args, "trained_surrogate", None))
     return out
 
 

@@ -21,6 +21,7 @@
 
 #include "../../include/gal_ode.h"
 #include "../../include/gal_tree.h"
+#include "../../include/gal_surrogate.h"
 #include "physics.hpp"
 #include "batch.hpp"
 
@@ -112,6 +113,56 @@ __global__ void event_kernel(int n, const int* __restrict__ kind, const int* __r
   }
 }
 
+// Compact copies of a wave's non-satellite evolve operations for the
+// surrogate: state, step size, times and node parameters (SoA, leading
+// dimension ldc).
+__global__ void gather_kernel(int n, const int* __restrict__ slot, int ld, const double* __restrict__ y,
+                              const double* __restrict__ h, const double* __restrict__ t0,
+                              const double* __restrict__ t1, const double* __restrict__ np, int ldc,
+                              double* __restrict__ yc, double* __restrict__ hc, double* __restrict__ t0c,
+                              double* __restrict__ t1c, double* __restrict__ npc) {
+  const int k = blockIdx.x * blockDim.x + threadIdx.x;
+  if (k >= n) return;
+  const size_t s = (size_t)slot[k];
+#pragma unroll
+  for (int v = 0; v < kNV; ++v) yc[(size_t)v * ldc + k] = y[(size_t)v * ld + s];
+  hc[k] = h[s];
+  t0c[k] = t0[k];
+  t1c[k] = t1[k];
+#pragma unroll
+  for (int j = 0; j < GAL_NPARAM_NODE; ++j) npc[(size_t)j * ldc + k] = np[(size_t)GAL_NPARAM_NODE * k + j];
+}
+
+__global__ void scatter_back_kernel(int n, const int* __restrict__ slot, int ldc, const double* __restrict__ yc,
+                                    const double* __restrict__ hc, const int* __restrict__ stc,
+                                    const double* __restrict__ scc, const int* __restrict__ acc, int ld,
+                                    double* __restrict__ y, double* __restrict__ h, int* __restrict__ status,
+                                    double* __restrict__ scale, unsigned long long* __restrict__ n_accepted) {
+  const int k = blockIdx.x * blockDim.x + threadIdx.x;
+  if (k >= n) return;
+  const size_t s = (size_t)slot[k];
+#pragma unroll
+  for (int v = 0; v < kNV; ++v) y[(size_t)v * ld + s] = yc[(size_t)v * ldc + k];
+  if (scale) {
+#pragma unroll
+    for (int v = 0; v < kNV; ++v) scale[(size_t)v * ld + s] = scc[(size_t)v * ldc + k];
+  }
+  h[s] = hc[k];
+  status[s] = stc[k];
+  if (acc[k]) atomicAdd(n_accepted, 1ull);
+}
+
+// Copy the states of a wave's evolve operations into a record (SoA, leading
+// dimension ldr, column r0 + k for operation k of the wave).
+__global__ void record_kernel(int n, const int* __restrict__ slot, int ld, const double* __restrict__ y, int ldr,
+                              int64_t r0, double* __restrict__ rec) {
+  const int k = blockIdx.x * blockDim.x + threadIdx.x;
+  if (k >= n) return;
+  const size_t s = (size_t)slot[k];
+#pragma unroll
+  for (int v = 0; v < kNV; ++v) rec[(size_t)v * ldr + (size_t)(r0 + k)] = y[(size_t)v * ld + s];
+}
+
 template <typename T>
 T* upload(const std::vector<T>& v, hipStream_t s, bool& ok) {
   if (v.empty() || !ok) return nullptr;
@@ -123,10 +174,29 @@ T* upload(const std::vector<T>& v, hipStream_t s, bool& ok) {
 
 }  // namespace
 
-extern "C" int gal_tree_evolve_d(gal_ode_batch* h, const gal_schedule* sched, int ld, double* y, double* scale,
-                                 int* status, double major_ratio, void* stream) {
+struct SurrogateRoute {
+  gal_mlp* m;
+  const float* norm;
+  const gal_surrogate_gate* gate;
+  int64_t* n_accepted;   // host out
+};
+
+struct Recording {
+  double* y0;            // device [GAL_NVAR * cap]: state before each evolve operation
+  double* y1;            // device [GAL_NVAR * cap]: state after it
+  double* t0;            // host [cap]
+  double* t1;            // host [cap]
+  double* node_par;      // host [GAL_NPARAM_NODE * cap], SoA with leading dimension cap
+  int64_t cap;
+  int64_t* n;            // host out: number of evolve operations recorded
+};
+
+static int tree_evolve(gal_ode_batch* h, const gal_schedule* sched, int ld, double* y, double* scale, int* status,
+                       double major_ratio, void* stream, const SurrogateRoute* sr, const Recording* rec = nullptr) {
   if (!h || !sched || ld < 1 || ld > h->n_max || !y || !status) return GAL_E_INVALID;
   if ((h->flags & GAL_FLAG_SCALES_PROVIDED) && !scale) return GAL_E_INVALID;
+  if (sr && (!sr->m || !sr->norm || !sr->gate)) return GAL_E_INVALID;
+  if (sr && sr->n_accepted) *sr->n_accepted = 0;
   int64_t n_ops = 0;
   int n_waves = 0;
   if (gal_schedule_size(sched, &n_ops, &n_waves) != GAL_OK) return GAL_E_INVALID;
@@ -135,22 +205,37 @@ extern "C" int gal_tree_evolve_d(gal_ode_batch* h, const gal_schedule* sched, in
   gal_schedule_ops(sched, ops.data());
   for (const gal_schedule_op& op : ops)
     if (op.a < 0 || op.a >= ld || op.b >= ld) return GAL_E_INVALID;
-  // Bucket the operations by wave (stable: within a wave the order of the
-  // schedule is kept, although operations of one wave touch distinct slots).
-  std::vector<int64_t> ev_count(n_waves + 1, 0), evt_count(n_waves + 1, 0);
+  // Per wave: evolve operations (with the surrogate, those of halos that are
+  // not satellites first) and event operations.  Operations of one wave
+  // touch distinct nodes, so their order within the wave is free.
+  std::vector<int64_t> ev_count(n_waves + 1, 0), evt_count(n_waves + 1, 0), ns_count(n_waves, 0);
   for (const gal_schedule_op& op : ops) {
-    if (op.kind == GAL_OP_EVOLVE) ev_count[op.wave + 1]++;
-    else if (op.kind == GAL_OP_PROMOTE || op.kind == GAL_OP_MERGE_GALAXY) evt_count[op.wave + 1]++;
+    if (op.kind == GAL_OP_EVOLVE) {
+      ev_count[op.wave + 1]++;
+      if (sr && op.node_par[GAL_NP_IS_SATELLITE] == 0.0) ns_count[op.wave]++;
+    } else if (op.kind == GAL_OP_PROMOTE || op.kind == GAL_OP_MERGE_GALAXY) {
+      evt_count[op.wave + 1]++;
+    }
   }
   for (int w = 0; w < n_waves; ++w) { ev_count[w + 1] += ev_count[w]; evt_count[w + 1] += evt_count[w]; }
   const size_t n_ev = (size_t)ev_count[n_waves], n_evt = (size_t)evt_count[n_waves];
+  if (rec && (!rec->y0 || !rec->y1 || !rec->t0 || !rec->t1 || !rec->node_par || (int64_t)n_ev > rec->cap))
+    return GAL_E_INVALID;
   std::vector<int> ev_slot(n_ev), evt_kind(n_evt), evt_a(n_evt), evt_b(n_evt);
   std::vector<double> ev_t0(n_ev), ev_t1(n_ev), ev_np(GAL_NPARAM_NODE * n_ev);
+  int64_t cap = 0;
   {
     std::vector<int64_t> pe(ev_count.begin(), ev_count.end() - 1), pt(evt_count.begin(), evt_count.end() - 1);
+    std::vector<int64_t> ps(n_waves);
+    for (int w = 0; w < n_waves; ++w) {
+      ps[w] = ev_count[w];                  // satellites follow the first ns_count[w] entries
+      pe[w] = ev_count[w] + ns_count[w];
+      cap = std::max(cap, ns_count[w]);
+    }
     for (const gal_schedule_op& op : ops) {
       if (op.kind == GAL_OP_EVOLVE) {
-        const int64_t k = pe[op.wave]++;
+        const bool front = sr && op.node_par[GAL_NP_IS_SATELLITE] == 0.0;
+        const int64_t k = front ? ps[op.wave]++ : pe[op.wave]++;
         ev_slot[k] = op.a;
         ev_t0[k] = op.t0;
         ev_t1[k] = op.t1;
@@ -162,6 +247,14 @@ extern "C" int gal_tree_evolve_d(gal_ode_batch* h, const gal_schedule* sched, in
         evt_b[k] = op.b;
       }
     }
+  }
+  if (rec) {
+    for (size_t k = 0; k < n_ev; ++k) {
+      rec->t0[k] = ev_t0[k];
+      rec->t1[k] = ev_t1[k];
+      for (int j = 0; j < GAL_NPARAM_NODE; ++j) rec->node_par[(size_t)j * rec->cap + k] = ev_np[GAL_NPARAM_NODE * k + j];
+    }
+    if (rec->n) *rec->n = (int64_t)n_ev;
   }
   hipStream_t s = (hipStream_t)stream;
   bool ok = true;
@@ -176,6 +269,20 @@ extern "C" int gal_tree_evolve_d(gal_ode_batch* h, const gal_schedule* sched, in
   ok = ok && hipMalloc(&d_t0, sizeof(double) * ld) == hipSuccess && hipMalloc(&d_t1, sizeof(double) * ld) == hipSuccess &&
        hipMalloc(&d_np, sizeof(double) * GAL_NPARAM_NODE * (size_t)ld) == hipSuccess &&
        hipMalloc(&d_h, sizeof(double) * ld) == hipSuccess;
+  // compact buffers of the surrogate route
+  double *c_y = nullptr, *c_h = nullptr, *c_t0 = nullptr, *c_t1 = nullptr, *c_np = nullptr, *c_sc = nullptr;
+  int *c_st = nullptr, *c_acc = nullptr;
+  unsigned long long* d_nacc = nullptr;
+  if (sr && cap > 0) {
+    const size_t C = (size_t)cap;
+    ok = ok && hipMalloc(&c_y, sizeof(double) * kNV * C) == hipSuccess &&
+         hipMalloc(&c_sc, sizeof(double) * kNV * C) == hipSuccess && hipMalloc(&c_h, sizeof(double) * C) == hipSuccess &&
+         hipMalloc(&c_t0, sizeof(double) * C) == hipSuccess && hipMalloc(&c_t1, sizeof(double) * C) == hipSuccess &&
+         hipMalloc(&c_np, sizeof(double) * GAL_NPARAM_NODE * C) == hipSuccess &&
+         hipMalloc(&c_st, sizeof(int) * C) == hipSuccess && hipMalloc(&c_acc, sizeof(int) * C) == hipSuccess &&
+         hipMalloc(&d_nacc, sizeof(unsigned long long)) == hipSuccess &&
+         hipMemsetAsync(d_nacc, 0, sizeof(unsigned long long), s) == hipSuccess;
+  }
   int rc = ok ? GAL_OK : GAL_E_NOMEM;
   if (rc == GAL_OK) {
     hipLaunchKernelGGL(fill_kernel, dim3((ld + 255) / 256), dim3(256), 0, s, ld, -1.0, d_h);
@@ -187,14 +294,36 @@ extern "C" int gal_tree_evolve_d(gal_ode_batch* h, const gal_schedule* sched, in
   }
   for (int w = 0; w < n_waves && rc == GAL_OK; ++w) {
     const int64_t e0 = ev_count[w], ne = ev_count[w + 1] - ev_count[w];
-    if (ne > 0) {
-      hipLaunchKernelGGL(scatter_kernel, dim3((unsigned)((ne + 255) / 256)), dim3(256), 0, s, (int)ne, d_ev_slot + e0,
-                         (const double*)d_ev_t0 + e0, (const double*)d_ev_t1 + e0,
-                         (const double*)d_ev_np + GAL_NPARAM_NODE * e0, ld, d_t0, d_t1, d_np);
-      rc = gal_ode_evolve_subset(h, ld, ld, y, d_np, d_t0, d_t1, d_h, scale, status, nullptr, d_ev_slot + e0, (int)ne,
+    const int64_t ns = sr ? ns_count[w] : 0;
+    if (rec && ne > 0)
+      hipLaunchKernelGGL(record_kernel, dim3((unsigned)((ne + 255) / 256)), dim3(256), 0, s, (int)ne,
+                         (const int*)d_ev_slot + e0, ld, (const double*)y, (int)rec->cap, e0, rec->y0);
+    if (ns > 0) {
+      const unsigned blocks = (unsigned)((ns + 255) / 256);
+      hipLaunchKernelGGL(gather_kernel, dim3(blocks), dim3(256), 0, s, (int)ns, (const int*)d_ev_slot + e0, ld,
+                         (const double*)y, (const double*)d_h, (const double*)d_ev_t0 + e0,
+                         (const double*)d_ev_t1 + e0, (const double*)d_ev_np + GAL_NPARAM_NODE * e0, (int)cap, c_y,
+                         c_h, c_t0, c_t1, c_np);
+      rc = gal_surrogate_evolve_d(h, sr->m, sr->norm, sr->gate, (int)ns, (int)cap, c_y, c_np, c_t0, c_t1, c_h, c_sc,
+                                  c_st, c_acc, nullptr, s);
+      if (rc != GAL_OK) break;
+      hipLaunchKernelGGL(scatter_back_kernel, dim3(blocks), dim3(256), 0, s, (int)ns, (const int*)d_ev_slot + e0,
+                         (int)cap, (const double*)c_y, (const double*)c_h, (const int*)c_st, (const double*)c_sc,
+                         (const int*)c_acc, ld, y, d_h, status, scale, d_nacc);
+      if (hipGetLastError() != hipSuccess) { rc = GAL_E_HIP; break; }
+    }
+    if (ne - ns > 0) {
+      const int64_t r0 = e0 + ns, nr = ne - ns;
+      hipLaunchKernelGGL(scatter_kernel, dim3((unsigned)((nr + 255) / 256)), dim3(256), 0, s, (int)nr, d_ev_slot + r0,
+                         (const double*)d_ev_t0 + r0, (const double*)d_ev_t1 + r0,
+                         (const double*)d_ev_np + GAL_NPARAM_NODE * r0, ld, d_t0, d_t1, d_np);
+      rc = gal_ode_evolve_subset(h, ld, ld, y, d_np, d_t0, d_t1, d_h, scale, status, nullptr, d_ev_slot + r0, (int)nr,
                                  s);
       if (rc != GAL_OK) break;
     }
+    if (rec && ne > 0)
+      hipLaunchKernelGGL(record_kernel, dim3((unsigned)((ne + 255) / 256)), dim3(256), 0, s, (int)ne,
+                         (const int*)d_ev_slot + e0, ld, (const double*)y, (int)rec->cap, e0, rec->y1);
     const int64_t v0 = evt_count[w], nv = evt_count[w + 1] - evt_count[w];
     if (nv > 0) {
       hipLaunchKernelGGL(event_kernel, dim3((unsigned)((nv + 255) / 256)), dim3(256), 0, s, (int)nv,
@@ -204,8 +333,37 @@ extern "C" int gal_tree_evolve_d(gal_ode_batch* h, const gal_schedule* sched, in
     }
   }
   if (hipStreamSynchronize(s) != hipSuccess && rc == GAL_OK) rc = GAL_E_HIP;
+  if (rc == GAL_OK && sr && sr->n_accepted && d_nacc) {
+    unsigned long long na = 0;
+    if (hipMemcpy(&na, d_nacc, sizeof(na), hipMemcpyDeviceToHost) != hipSuccess) rc = GAL_E_HIP;
+    *sr->n_accepted = (int64_t)na;
+  }
   hipFree(d_ev_slot); hipFree(d_ev_t0); hipFree(d_ev_t1); hipFree(d_ev_np);
   hipFree(d_evt_kind); hipFree(d_evt_a); hipFree(d_evt_b);
   hipFree(d_t0); hipFree(d_t1); hipFree(d_np); hipFree(d_h);
+  hipFree(c_y); hipFree(c_sc); hipFree(c_h); hipFree(c_t0); hipFree(c_t1); hipFree(c_np); hipFree(c_st);
+  hipFree(c_acc); hipFree(d_nacc);
   return rc;
+}
+
+extern "C" int gal_tree_evolve_d(gal_ode_batch* h, const gal_schedule* sched, int ld, double* y, double* scale,
+                                 int* status, double major_ratio, void* stream) {
+  return tree_evolve(h, sched, ld, y, scale, status, major_ratio, stream, nullptr);
+}
+
+extern "C" int gal_tree_evolve_surrogate_d(gal_ode_batch* h, gal_mlp* m, const float* norm,
+                                           const gal_surrogate_gate* gate, const gal_schedule* sched, int ld,
+                                           double* y, double* scale, int* status, double major_ratio,
+                                           int64_t* n_accepted, void* stream) {
+  SurrogateRoute sr{m, norm, gate, n_accepted};
+  return tree_evolve(h, sched, ld, y, scale, status, major_ratio, stream, &sr);
+}
+
+extern "C" int gal_tree_evolve_record_d(gal_ode_batch* h, const gal_schedule* sched, int ld, double* y, double* scale,
+                                        int* status, double major_ratio, double* rec_y0, double* rec_y1,
+                                        double* rec_t0, double* rec_t1, double* rec_node_par, int64_t capacity,
+                                        int64_t* n_recorded, void* stream) {
+  if (capacity < 1 || capacity > INT32_MAX) return GAL_E_INVALID;
+  Recording rec{rec_y0, rec_y1, rec_t0, rec_t1, rec_node_par, capacity, n_recorded};
+  return tree_evolve(h, sched, ld, y, scale, status, major_ratio, stream, nullptr, &rec);
 }

@@ -186,6 +186,47 @@ class PairSet:
         return int(self.t0.shape[0])
 
 
+def generate_tree_pairs(physics, table, max_pairs: int, seed: int = 0, trees_per_forest: int = 1024,
+                        log_root_mass=(10.0, 12.0), z_start: float = 5.0, mass_resolution: float = 1e9,
+                        tol_abs: float = 1e-6, tol_rel: float = 1e-3, device="cuda:0") -> PairSet:
+    """Training pairs from merger-tree trajectories: Parkinson-Cole-Helly
+    forests (roots log-uniform in 10^log_root_mass Msun at z=0, the given
+    resolution, evolved from z_start in the reference's walk order by the
+    FP64 integrator), one forest per seed from `seed` on, until max_pairs
+    intervals of non-satellite halos are recorded (the states before and
+    after each interval, its times and halo parameters)."""
+    import torch
+    from . import trees as tr
+    cosmo = tr.cosmology(physics)
+    t_end = float(wl.scale_factor_to_cosmic_time(1.0))
+    t_beg = float(wl.scale_factor_to_cosmic_time(1.0 / (1.0 + z_start)))
+    parts = {"y0": [], "y1": [], "np": [], "t0": [], "t1": []}
+    total, k = 0, 0
+    while total < max_pairs:
+        rng = np.random.Generator(np.random.PCG64(seed + k))
+        roots = 10.0 ** rng.uniform(log_root_mass[0], log_root_mass[1], trees_per_forest)
+        forest = tr.Forest.build_pch(trees_per_forest, roots, t_end, cosmo, tr.pch_params(mass_resolution, t_beg),
+                                     seed=seed + k)
+        wp = tr.walk_params(t_end)
+        sched = tr.Schedule(forest, cosmo, wp)
+        ev = NodeEvolverBatched(forest.n_nodes, physics, table, odeToleranceAbsolute=tol_abs,
+                                odeToleranceRelative=tol_rel)
+        y = torch.from_numpy(forest.initial_state(cosmo, wp)).to(device)
+        y0, y1, npar, t0, t1, _ = tr.record_forest(ev, sched, y)
+        keep = (npar[4] == 0) & (t1 > t0)
+        idx = torch.nonzero(keep).flatten()[: max_pairs - total]
+        parts["y0"].append(y0[:, idx])
+        parts["y1"].append(y1[:, idx])
+        parts["np"].append(npar[:, idx])
+        parts["t0"].append(t0[idx])
+        parts["t1"].append(t1[idx])
+        total += int(idx.numel())
+        k += 1
+        del ev, y, y0, y1
+    cat = lambda key, dim: torch.cat(parts[key], dim=dim).contiguous()
+    return PairSet(cat("y0", 1), cat("y1", 1), cat("np", 1), cat("t0", 0), cat("t1", 0))
+
+
 def generate_pairs(evolver: NodeEvolverBatched, hist: wl.HaloHistories, max_pairs: int, seed: int = 0,
                    device="cuda:0") -> PairSet:
     """Evolve every history interval by interval to z=0 with the FP64

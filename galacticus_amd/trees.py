@@ -111,10 +111,14 @@ def _bind(L):
     L.gal_forest_initial_state.argtypes = [vp, ctypes.POINTER(Cosmology), ctypes.POINTER(WalkParams),
                                            ctypes.c_double, ctypes.c_double, c_int, dp]
     L.gal_tree_evolve_d.argtypes = [vp, vp, c_int, vp, vp, vp, ctypes.c_double, vp]
+    L.gal_tree_evolve_surrogate_d.argtypes = [vp, vp, vp, ctypes.POINTER(_lib.Gate), vp, c_int, vp, vp, vp,
+                                              ctypes.c_double, ctypes.POINTER(ctypes.c_int64), vp]
+    L.gal_tree_evolve_record_d.argtypes = [vp, vp, c_int, vp, vp, vp, ctypes.c_double, vp, vp, vp, vp, vp,
+                                           ctypes.c_int64, ctypes.POINTER(ctypes.c_int64), vp]
     for name in ("gal_forest_build_pch", "gal_forest_from_arrays", "gal_forest_size", "gal_forest_arrays",
                  "gal_forest_destroy", "gal_cosmology_sigma", "gal_cosmology_delta_critical", "gal_schedule_build",
                  "gal_schedule_size", "gal_schedule_ops", "gal_schedule_destroy", "gal_forest_initial_state",
-                 "gal_tree_evolve_d"):
+                 "gal_tree_evolve_d", "gal_tree_evolve_surrogate_d", "gal_tree_evolve_record_d"):
         getattr(L, name).restype = c_int
     L._trees_bound = True
     return L
@@ -337,9 +341,13 @@ def evolve_forests(evolvers, schedules, ys, streams, major_ratio=0.25):
     return out
 
 
-def evolve_forest(evolver, schedule: Schedule, y, scale=None, status=None, major_ratio=0.25, stream=None):
+def evolve_forest(evolver, schedule: Schedule, y, scale=None, status=None, major_ratio=0.25, stream=None,
+                  surrogate=None):
     """Run the schedule on device state y (torch float64, (30, n_nodes));
-    returns the per-node status tensor."""
+    returns the per-node status tensor.  surrogate: a SurrogateEvolver whose
+    network and gate take the evolve operations of non-satellite halos
+    (gal_tree_evolve_surrogate_d); its count of accepted intervals is left in
+    surrogate.tree_accepted."""
     import torch
     n = int(y.shape[1])
     if status is None:
@@ -348,7 +356,46 @@ def evolve_forest(evolver, schedule: Schedule, y, scale=None, status=None, major
         scale = torch.zeros_like(y)
     L = _bind(evolver.L)
     s = ctypes.c_void_p(0 if stream is None else stream.cuda_stream)
-    _lib.check(L.gal_tree_evolve_d(evolver.handle, schedule.handle, n, ctypes.c_void_p(y.data_ptr()),
-                                   ctypes.c_void_p(scale.data_ptr()), ctypes.c_void_p(status.data_ptr()),
-                                   float(major_ratio), s), "gal_tree_evolve_d")
+    if surrogate is None:
+        _lib.check(L.gal_tree_evolve_d(evolver.handle, schedule.handle, n, ctypes.c_void_p(y.data_ptr()),
+                                       ctypes.c_void_p(scale.data_ptr()), ctypes.c_void_p(status.data_ptr()),
+                                       float(major_ratio), s), "gal_tree_evolve_d")
+        return status
+    n_acc = ctypes.c_int64(0)
+    g = surrogate.gate.c_struct()
+    _lib.check(L.gal_tree_evolve_surrogate_d(evolver.handle, surrogate.mlp.handle,
+                                             ctypes.c_void_p(surrogate.norm.data_ptr()), ctypes.byref(g),
+                                             schedule.handle, n, ctypes.c_void_p(y.data_ptr()),
+                                             ctypes.c_void_p(scale.data_ptr()), ctypes.c_void_p(status.data_ptr()),
+                                             float(major_ratio), ctypes.byref(n_acc), s),
+               "gal_tree_evolve_surrogate_d")
+    surrogate.tree_accepted = int(n_acc.value)
     return status
+
+
+def record_forest(evolver, schedule: Schedule, y, major_ratio=0.25):
+    """Run the schedule like evolve_forest and return every evolve operation's
+    trajectory piece: (y_before, y_after, node_par, t0, t1) as device tensors
+    (30 x n, 30 x n, 5 x n, n, n), in execution order, and the status."""
+    import torch
+    n = int(y.shape[1])
+    n_ev = int((schedule.ops["kind"] == OP_EVOLVE).sum())
+    cap = max(n_ev, 1)
+    dev = y.device
+    status = torch.zeros(n, dtype=torch.int32, device=dev)
+    scale = torch.zeros_like(y)
+    y0 = torch.empty((gp.NVAR, cap), dtype=torch.float64, device=dev)
+    y1 = torch.empty_like(y0)
+    t0 = np.zeros(cap)
+    t1 = np.zeros(cap)
+    npar = np.zeros((gp.NPARAM_NODE, cap))
+    n_rec = ctypes.c_int64(0)
+    L = _bind(evolver.L)
+    _lib.check(L.gal_tree_evolve_record_d(evolver.handle, schedule.handle, n, ctypes.c_void_p(y.data_ptr()),
+                                          ctypes.c_void_p(scale.data_ptr()), ctypes.c_void_p(status.data_ptr()),
+                                          float(major_ratio), ctypes.c_void_p(y0.data_ptr()),
+                                          ctypes.c_void_p(y1.data_ptr()), _dp(t0), _dp(t1), _dp(npar), cap,
+                                          ctypes.byref(n_rec), None), "gal_tree_evolve_record_d")
+    k = int(n_rec.value)
+    d = lambda a: torch.from_numpy(np.ascontiguousarray(a)).to(dev)
+    return y0[:, :k], y1[:, :k], d(npar[:, :k]), d(t0[:k]), d(t1[:k]), status

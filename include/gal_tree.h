@@ -119,6 +119,31 @@ int gal_forest_initial_state(const gal_forest* f, const gal_cosmology* cosmo, co
 int gal_tree_evolve_d(gal_ode_batch* h, const gal_schedule* s, int ld, double* y, double* scale, int* status,
                       double major_ratio, void* stream);
 
+/* The same with the learned surrogate in front of the integrator
+   (include/gal_surrogate.h): in every wave the evolve operations of halos
+   that are not satellites go through gal_surrogate_evolve_d (network, gate,
+   integrator for the rejected), those of satellites -- outside the
+   network's training distribution -- straight to the integrator.
+   *n_accepted (host, may be NULL) counts the intervals the surrogate took.
+   Needs gal_surrogate.h's types. */
+struct gal_mlp;
+struct gal_surrogate_gate;
+
+/* gal_tree_evolve_d that also records every evolve operation, in execution
+   order: the node's state before (rec_y0, device) and after (rec_y1, device)
+   the interval, SoA [GAL_NVAR*capacity]; its times (rec_t0, rec_t1, host
+   [capacity]) and node parameters (rec_node_par, host
+   [GAL_NPARAM_NODE*capacity], SoA).  *n_recorded (host) = number of evolve
+   operations; capacity must hold them all.  The surrogate's training pairs
+   come from these trajectories. */
+int gal_tree_evolve_record_d(gal_ode_batch* h, const gal_schedule* s, int ld, double* y, double* scale,
+                             int* status, double major_ratio, double* rec_y0, double* rec_y1, double* rec_t0,
+                             double* rec_t1, double* rec_node_par, int64_t capacity, int64_t* n_recorded,
+                             void* stream);
+int gal_tree_evolve_surrogate_d(gal_ode_batch* h, struct gal_mlp* m, const float* norm,
+                                const struct gal_surrogate_gate* gate, const gal_schedule* s, int ld, double* y,
+                                double* scale, int* status, double major_ratio, int64_t* n_accepted, void* stream);
+
 #ifdef __cplusplus
 }
 #endif

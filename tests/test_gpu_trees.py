@@ -152,3 +152,75 @@ def test_concurrent_forests_equal_one_forest(torch_cuda, cooling_table):
     st_split = np.concatenate([s.cpu().numpy() for s in sts])
     assert np.array_equal(y_split, yd.cpu().numpy())
     assert np.array_equal(st_split, st)
+
+
+def test_surrogate_route_rejecting_everything_equals_plain_forest(torch_cuda, cooling_table):
+    """With a gate that rejects every step, the surrogate route of the tree
+    executor (gather, network, gate, integrator on the compact copies,
+    scatter back; satellites straight to the integrator) ends in exactly the
+    plain executor's state, step sizes and statuses."""
+    torch = torch_cuda
+    from galacticus_amd.evolver import NodeEvolverBatched
+    from galacticus_amd.surrogate import SurrogateMLP, SurrogateEvolver, GateConfig
+    physics = gp.physics_params()
+    cosmo = tr.cosmology(physics)
+    forest = tr.Forest.build_pch(6, np.geomspace(1e11, 1e12, 6), T0, cosmo, tr.pch_params(2e10, T_Z5), seed=3)
+    wp = tr.walk_params(T0, merging_multiplier=0.2)
+    sched = tr.Schedule(forest, cosmo, wp)
+    y0 = forest.initial_state(cosmo, wp)
+    ev = NodeEvolverBatched(forest.n_nodes, physics, cooling_table)
+    ya = torch.from_numpy(y0.copy()).cuda()
+    sa = torch.zeros_like(ya)
+    st_a = tr.evolve_forest(ev, sched, ya, scale=sa).cpu().numpy()
+    norm = torch.zeros((4, 32), dtype=torch.float32, device="cuda")
+    norm[1] = 1.0
+    norm[3] = 1.0
+    se = SurrogateEvolver(ev, SurrogateMLP(1024, seed=1), norm, GateConfig(error_max=1e-30))
+    yb = torch.from_numpy(y0.copy()).cuda()
+    sb = torch.zeros_like(yb)
+    st_b = tr.evolve_forest(ev, sched, yb, scale=sb, surrogate=se).cpu().numpy()
+    n_sat = int((sched.ops["node_par"][:, 4] != 0).sum())
+    print("nodes", forest.n_nodes, "evolves", int((sched.ops["kind"] == 0).sum()), "satellite evolves", n_sat,
+          "accepted", se.tree_accepted, flush=True)
+    assert se.tree_accepted == 0
+    assert np.array_equal(yb.cpu().numpy(), ya.cpu().numpy())
+    assert np.array_equal(sb.cpu().numpy(), sa.cpu().numpy())
+    assert np.array_equal(st_b, st_a)
+
+
+def test_recorded_intervals_are_the_forest_trajectory(torch_cuda, cooling_table):
+    """The recording executor leaves the forest exactly as the plain one, and
+    every recorded interval is a piece of the trajectory: re-evolving its
+    recorded start state over its interval lands on its recorded end state
+    within the ODE tolerance (bit-exact is not expected: the recorded run
+    reused step sizes from earlier intervals)."""
+    torch = torch_cuda
+    from galacticus_amd.evolver import NodeEvolverBatched
+    physics = gp.physics_params()
+    cosmo = tr.cosmology(physics)
+    forest = tr.Forest.build_pch(6, np.geomspace(1e11, 1e12, 6), T0, cosmo, tr.pch_params(2e10, T_Z5), seed=3)
+    wp = tr.walk_params(T0, merging_multiplier=0.2)
+    sched = tr.Schedule(forest, cosmo, wp)
+    y0 = forest.initial_state(cosmo, wp)
+    ev = NodeEvolverBatched(forest.n_nodes, physics, cooling_table, odeToleranceAbsolute=1e-6,
+                            odeToleranceRelative=1e-3)
+    ya = torch.from_numpy(y0.copy()).cuda()
+    tr.evolve_forest(ev, sched, ya)
+    yb = torch.from_numpy(y0.copy()).cuda()
+    r0, r1, npar, t0, t1, st = tr.record_forest(ev, sched, yb)
+    n = int(t0.shape[0])
+    print("recorded", n, "of", int((sched.ops["kind"] == 0).sum()), flush=True)
+    assert n == int((sched.ops["kind"] == 0).sum())
+    assert torch.equal(ya, yb) and bool((st == 0).all())
+    ev2 = NodeEvolverBatched(n, physics, cooling_table, odeToleranceAbsolute=1e-6, odeToleranceRelative=1e-3)
+    y = r0.clone()
+    sc = torch.zeros_like(y)
+    st2 = ev2.evolve_device(y, npar.contiguous(), t0, t1, -torch.ones(n, dtype=torch.float64, device="cuda"), scale=sc)
+    assert bool((st2 == 0).all())
+    mass = [V[k] for k in ("HOT_MASS", "DISK_MASS_GAS", "DISK_MASS_STELLAR", "SPHEROID_MASS_GAS",
+                           "SPHEROID_MASS_STELLAR", "BH_MASS")]
+    got, ref, s = y[mass].cpu().numpy(), r1[mass].cpu().numpy(), sc[mass].cpu().numpy()
+    tol = 1e-3 * np.abs(ref) + 1e-6 * np.abs(s)
+    worst = (np.abs(got - ref) / np.maximum(tol, 1e-300)).max()
+    print("max |dy| / tolerance", worst, flush=True)
+    assert worst < 50.0
