@@ -224,3 +224,45 @@ def test_recorded_intervals_are_the_forest_trajectory(torch_cuda, cooling_table)
     worst = (np.abs(got - ref) / np.maximum(tol, 1e-300)).max()
     print("max |dy| / tolerance", worst, flush=True)
     assert worst < 50.0
+
+
+def test_64_tree_workload_against_the_cpu_tree_runner(torch_cuda, cooling_table):
+    """The 64-tree workload at full size (PCH, seed 0, roots log-uniform
+    1e10-1e12 Msun, resolution 1e9 Msun, z=5 -> 0): the GPU wave executor
+    against the CPU oracle's tree runner on the same schedule.  Both take the
+    same steps almost everywhere (median relative difference of the masses
+    below 1e-9); where they do not they agree to the ODE tolerance; the z=0
+    stellar masses of the roots agree to 1e-6."""
+    torch = torch_cuda
+    from galacticus_amd.evolver import NodeEvolverBatched
+    physics = gp.physics_params()
+    cosmo = tr.cosmology(physics)
+    rng = np.random.Generator(np.random.PCG64(0))
+    forest = tr.Forest.build_pch(64, 10.0 ** rng.uniform(10.0, 12.0, 64), T0, cosmo, tr.pch_params(1e9, T_Z5), seed=0)
+    wp = tr.walk_params(T0)
+    sched = tr.Schedule(forest, cosmo, wp)
+    y0 = forest.initial_state(cosmo, wp)
+    print("nodes", forest.n_nodes, "operations", sched.n_ops, "waves", sched.n_waves, flush=True)
+    ev = NodeEvolverBatched(forest.n_nodes, physics, cooling_table, odeToleranceAbsolute=1e-6,
+                            odeToleranceRelative=1e-3)
+    yd = torch.from_numpy(y0.copy()).cuda()
+    sd = torch.zeros_like(yd)
+    st = tr.evolve_forest(ev, sched, yd, scale=sd).cpu().numpy()
+    yg, sc = yd.cpu().numpy(), sd.cpu().numpy()
+    yc, _, stc, bad = oracle.tree_run(sched.ops, y0, -np.ones(forest.n_nodes), forest.tree_of_node(),
+                                      forest.n_trees, physics, cooling_table, 1e-6, 1e-3, gp.FLAG_REUSE_STEP_SIZE,
+                                      0.25, root_tol_rel=1e-15, nthreads=8)
+    assert bad == 0 and (st == 0).all() and (stc == 0).all()
+    mass = [V[k] for k in ("HOT_MASS", "DISK_MASS_GAS", "DISK_MASS_STELLAR", "SPHEROID_MASS_GAS",
+                           "SPHEROID_MASS_STELLAR", "BH_MASS")]
+    ref = np.abs(yc[mass])
+    big = ref > 1e-6 * np.abs(yc[V["HOT_MASS"]]).max()
+    rel = np.abs(yg[mass] - yc[mass])[big] / ref[big]
+    tol = 1e-3 * np.abs(yc[mass]) + 1e-6 * np.abs(sc[mass])
+    worst = (np.abs(yg[mass] - yc[mass]) / np.maximum(tol, 1e-300)).max()
+    print("median rel", np.median(rel), "max |dy|/tol", worst, flush=True)
+    assert np.median(rel) < 1e-9 and worst < 50.0
+    r = forest.roots()
+    mg = yg[V["DISK_MASS_STELLAR"], r] + yg[V["SPHEROID_MASS_STELLAR"], r]
+    mc = yc[V["DISK_MASS_STELLAR"], r] + yc[V["SPHEROID_MASS_STELLAR"], r]
+    assert np.allclose(mg, mc, rtol=1e-6, atol=0)
