@@ -42,8 +42,8 @@ FP64_PEAK_TFLOPS = 78.6
 # counter scaled by what it reads for one elementwise kernel that reads and
 # writes 1 GiB (FETCH_SIZE x 2.0000, WRITE_SIZE x 1.0000).
 HBM_TRAFFIC_1E5 = {
-    (64, 1): (228573.5 * 2.0 + 212418.6 * 1.0) * 1024,
-    (64, 2): (323554.8 * 2.0 + 689342.9 * 1.0) * 1024,
+    (64, 1): (228962.6 * 2.0 + 218809.2 * 1.0) * 1024,
+    (64, 2): (330335.9 * 2.0 + 707914.9 * 1.0) * 1024,
 }
 
 
