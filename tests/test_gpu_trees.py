@@ -64,6 +64,18 @@ def test_closed_box_tree(torch_cuda, cooling_table):
               "DISK_ABUNDANCES_STELLAR": 2.8814957e9}
     for k, v in expect.items():
         assert abs(root[V[k]] / v - 1.0) < 1e-2, (k, root[V[k]], v)
+    # the same known answers read back from the Galacticus-layout output, by
+    # the property names and output index of test-reproducibility.py
+    import tempfile
+    from galacticus_amd import output as go
+    with tempfile.TemporaryDirectory() as d:
+        path = os.path.join(d, "closedBox.npz")
+        go.write_output(path, [go.output_group(forest, sched, yd.cpu().numpy(), forest.time[0])])
+        f = go.read_output(path)
+    for prop, v in (("diskMassGas", 9.0717953e9), ("diskMassStellar", 9.0928205e10),
+                    ("diskAbundancesGasMetals", 9.0717953e8), ("diskAbundancesStellarMetals", 2.8814957e9)):
+        got = f[f"Outputs/Output1/nodeData/{prop}"]
+        assert got.shape == (1,) and abs(got[0] / v - 1.0) < 1e-2, (prop, got, v)
 
 
 def _oracle_apply(sched, y0, physics, table, tol_abs, tol_rel):

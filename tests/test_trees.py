@@ -274,3 +274,33 @@ def test_forest_split_keeps_trees(cosmo):
     key = lambda o: sorted(zip(o["kind"].tolist(), o["a"].tolist(), o["b"].tolist(), o["t0"].tolist(),
                                o["t1"].tolist()))
     assert key(ops) == key(whole)
+
+
+def test_output_layout_and_tree_contiguity(cosmo, tmp_path):
+    """The Galacticus-layout output: one entry per surviving node, trees in
+    contiguous blocks located by mergerTreeStartIndex/Count (the check of
+    testSuite/test-output-tree-contiguousity.py restated), properties equal
+    to the state, promoted children and merged satellites absent."""
+    from galacticus_amd import output as go
+    t0 = float(wl.scale_factor_to_cosmic_time(1.0))
+    t5 = float(wl.scale_factor_to_cosmic_time(1.0 / 6.0))
+    forest = tr.Forest.build_pch(5, np.geomspace(3e10, 3e11, 5), t0, cosmo, tr.pch_params(5e9, t5), seed=6)
+    wp = tr.walk_params(t0, merging_multiplier=0.2)
+    sched = tr.Schedule(forest, cosmo, wp)
+    y = np.random.default_rng(0).uniform(1.0, 2.0, (30, forest.n_nodes))
+    path = str(tmp_path / "out.npz")
+    go.write_output(path, [go.output_group(forest, sched, y, t0, 1.0)])
+    f = go.read_output(path)
+    out = f["Outputs"]["Output1"]
+    nd = out["nodeData"]
+    tree_by_node = nd["mergerTreeIndex"]
+    for idx, start, count in zip(out["mergerTreeIndex"], out["mergerTreeStartIndex"], out["mergerTreeCount"]):
+        assert (tree_by_node[start:start + count] == idx).all()
+    assert out["mergerTreeCount"].sum() == len(nd["nodeIndex"])
+    nodes = nd["nodeIndex"]
+    assert set(forest.roots().tolist()) <= set(nodes.tolist())
+    k = sched.ops["kind"]
+    removed = set(sched.ops["a"][(k == tr.OP_PROMOTE) | (k == tr.OP_MERGE_GALAXY)].tolist())
+    assert not (removed & set(nodes.tolist()))
+    assert np.array_equal(f["Outputs/Output1/nodeData/diskMassStellar"], y[wl.V["DISK_MASS_STELLAR"], nodes])
+    assert (nd["nodeIsIsolated"][np.isin(nodes, forest.roots())] == 1).all()
