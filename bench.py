@@ -660,6 +660,10 @@ def main():
             "traffic_unit": "bytes per launch (rocprofv3 counters, calibrated)",
             "flop_per_attempt": flop_step,
             "attempts_per_launch": attempts,
+            "accepted_steps_per_launch": int(cnt[1].sum()),
+            "rhs_evaluations_per_launch": int(cnt[2].sum()),
+            "rhs_evaluations_per_s": int(cnt[2].sum()) * args.steps / wall,
+            "accepted_steps_per_s": int(cnt[1].sum()) * args.steps / wall,
             "kernel_ms": kernel_ms,
             "aggregate_tflops": aggregate_tflops,
             # node state in and out of HBM: y, node parameters, t0, t1, h in;

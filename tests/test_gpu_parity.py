@@ -212,3 +212,45 @@ def test_launch_variants_bit_identical(torch_cuda, cooling_table):
         assert torch.equal(out[0][0], o[0]) and torch.equal(out[0][1], o[1]) and torch.equal(out[0][2], o[2])
     shape = ev.autotune(d(b.y), d(b.node_par), d(b.t_start), d(b.t_end), d(b.h_step))
     assert shape in ((64, 1), (64, 2), (64, 4))
+
+
+def test_evolve_captured_in_a_hip_graph(torch_cuda, cooling_table):
+    """The batch evolve (claim order by node index) records into a hipGraph on
+    the caller's stream -- two kernels, no host synchronisation, no memset --
+    and every replay, with eager work and synchronisations in between,
+    reproduces the eager result bit for bit."""
+    torch = torch_cuda
+    from galacticus_amd.evolver import NodeEvolverBatched
+    physics = gp.physics_params()
+    b = wl.isolated_halos(3000, seed=31, params=physics)
+    ev = NodeEvolverBatched(b.n, physics, cooling_table, odeToleranceAbsolute=1e-6, odeToleranceRelative=1e-3,
+                            cost_order=False)
+    d = lambda a: torch.from_numpy(np.ascontiguousarray(a)).cuda()
+    npar, t0, t1 = d(b.node_par), d(b.t_start), d(b.t_end)
+    y_ref, h_ref = d(b.y), d(b.h_step)
+    st_ref = ev.evolve_device(y_ref, npar, t0, t1, h_ref)
+    torch.cuda.synchronize()
+    y_s, h_s = d(b.y), d(b.h_step)
+    sc_s = torch.zeros_like(y_s)
+    st_s = torch.zeros(b.n, dtype=torch.int32, device="cuda")
+    side = torch.cuda.Stream()
+    side.wait_stream(torch.cuda.current_stream())
+    with torch.cuda.stream(side):        # warm-up on the capture stream
+        ev.evolve_device(y_s, npar, t0, t1, h_s, scale=sc_s, status=st_s, stream=side)
+    torch.cuda.current_stream().wait_stream(side)
+    torch.cuda.synchronize()
+    g = torch.cuda.CUDAGraph()
+    with torch.cuda.graph(g):
+        ev.evolve_device(y_s, npar, t0, t1, h_s, scale=sc_s, status=st_s, stream=torch.cuda.current_stream())
+    y_in, h_in = d(b.y), d(b.h_step)
+    for rep in range(3):
+        y_s.copy_(y_in)
+        h_s.copy_(h_in)
+        g.replay()
+        torch.cuda.synchronize()
+        assert torch.equal(y_s, y_ref) and torch.equal(h_s, h_ref) and torch.equal(st_s, st_ref), rep
+        # eager work between replays
+        y_e, h_e = d(b.y), d(b.h_step)
+        ev.evolve_device(y_e, npar, t0, t1, h_e)
+        torch.cuda.synchronize()
+        assert torch.equal(y_e, y_ref)
