@@ -38,8 +38,10 @@
  *   Variables with zero rate are carried over, the spin angular momentum is
  *   advanced analytically (its rate is the node's constant spin rate).
  * Normalisation array norm (device fp32, 4 x 32):
- *   [input mean | input std | target mean | target std]; inputs and targets
- *   are standardised with it before they reach the network.
+ *   [input mean | input std | target centre m | target scale sigma].  Inputs
+ *   are standardised, (x - mean) / std.  A target residual r is presented to
+ *   the network as asinh((r - m) / sigma), and a network output o decodes to
+ *   the residual m + sigma sinh(o) (o clipped to +-20).
  */
 #ifndef GAL_SURROGATE_H
 #define GAL_SURROGATE_H
