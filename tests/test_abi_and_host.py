@@ -24,7 +24,7 @@ def _declared_functions():
     for hdr in glob.glob(os.path.join(ROOT, "include", "*.h")):
         src = open(hdr).read()
         src = re.sub(r"/\*.*?\*/", "", src, flags=re.S)
-        names += re.findall(r"^\s*(?:int|void|size_t|double)\s+(gal_\w+)\s*\(", src, flags=re.M)
+        names += re.findall(r"^\s*(?:int64_t|int|void|size_t|double)\s+(gal_\w+)\s*\(", src, flags=re.M)
     return sorted(set(names))
 
 
