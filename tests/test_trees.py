@@ -304,3 +304,42 @@ def test_output_layout_and_tree_contiguity(cosmo, tmp_path):
     assert not (removed & set(nodes.tolist()))
     assert np.array_equal(f["Outputs/Output1/nodeData/diskMassStellar"], y[wl.V["DISK_MASS_STELLAR"], nodes])
     assert (nd["nodeIsIsolated"][np.isin(nodes, forest.roots())] == 1).all()
+
+
+@pytest.mark.parametrize("major", [False, True])
+def test_merge_galaxy_simple_hand_cases(major):
+    """mergerMassMovementsSimple with spheroid destinations
+    (satellites/merging/mass_movements/simple.F90:208-290), by hand: the
+    satellite's cold gas and stars (and their metals, formed-star tallies and
+    angular momentum) land in the host spheroid, its circumgalactic
+    reservoirs and black hole join the host's; in a major merger (smaller
+    galaxy at least 0.25 of the larger) the host disk moves to the spheroid
+    too; the satellite is left empty."""
+    V = wl.V
+    y = np.zeros((30, 2))
+    sat, host = 0, 1
+    y[V["DISK_MASS_GAS"], sat], y[V["DISK_MASS_STELLAR"], sat], y[V["SPHEROID_MASS_STELLAR"], sat] = 1.0, 2.0, 0.5
+    y[V["DISK_ABUNDANCES_GAS"], sat], y[V["HOT_MASS"], sat], y[V["BH_MASS"], sat] = 0.01, 3.0, 1e-3
+    y[V["DISK_MASS_GAS"], host], y[V["DISK_MASS_STELLAR"], host] = 4.0, (6.0 if major else 40.0)
+    y[V["SPHEROID_MASS_GAS"], host], y[V["HOT_MASS"], host], y[V["BH_MASS"], host] = 0.2, 10.0, 2e-3
+    before = y.copy()
+    ot.merge_galaxy_simple(y, sat, host, 0.25)
+    m_sat = 1.0 + 2.0 + 0.5
+    m_host = 4.0 + (6.0 if major else 40.0) + 0.2
+    assert (min(m_sat, m_host) >= 0.25 * max(m_sat, m_host)) == major
+    assert np.all(y[:, sat][[V[k] for k in ("DISK_MASS_GAS", "DISK_MASS_STELLAR", "SPHEROID_MASS_STELLAR",
+                                             "DISK_ABUNDANCES_GAS", "HOT_MASS", "BH_MASS")]] == 0.0)
+    assert y[V["HOT_MASS"], host] == 13.0 and y[V["BH_MASS"], host] == 3e-3
+    if major:
+        assert y[V["SPHEROID_MASS_GAS"], host] == 0.2 + 1.0 + 4.0
+        assert y[V["SPHEROID_MASS_STELLAR"], host] == 0.5 + 2.0 + 6.0
+        assert y[V["DISK_MASS_GAS"], host] == 0.0 and y[V["DISK_MASS_STELLAR"], host] == 0.0
+    else:
+        assert y[V["SPHEROID_MASS_GAS"], host] == 0.2 + 1.0
+        assert y[V["SPHEROID_MASS_STELLAR"], host] == 0.5 + 2.0
+        assert y[V["DISK_MASS_GAS"], host] == 4.0 and y[V["DISK_MASS_STELLAR"], host] == 40.0
+    assert y[V["SPHEROID_ABUNDANCES_GAS"], host] == 0.01
+    # total baryons conserved
+    mass = [V[k] for k in ("HOT_MASS", "DISK_MASS_GAS", "DISK_MASS_STELLAR", "SPHEROID_MASS_GAS",
+                           "SPHEROID_MASS_STELLAR", "BH_MASS")]
+    assert np.isclose(y[mass].sum(), before[mass].sum(), rtol=1e-15)
