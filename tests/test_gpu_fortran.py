@@ -52,3 +52,55 @@ def test_fortran_binding_evolves_a_batch(tmp_path, cooling_table):
                                        1e-3, gp.FLAG_REUSE_STEP_SIZE, root_tol_rel=1e-15)
     assert (sto == stf).all()
     assert (np.abs(yf - yo) <= 10.0 * (1e-3 * np.abs(yo) + 1e-6 * sco)).all()
+
+
+FOREST_EXE = os.path.join(ROOT, "galacticus_amd", "fortran", "build", "evolve_forest_file")
+
+
+def test_fortran_forest_driver_equals_python_driver(tmp_path, cooling_table):
+    """A Fortran host program builds a PCH forest, records the walk, keeps the
+    forest state in device memory and runs the schedule on the GPU through
+    the ISO_C_BINDING modules; the same forest driven from Python ends in the
+    same state, bit for bit, with the same statuses."""
+    import torch
+    from galacticus_amd import trees as tr
+    from galacticus_amd.evolver import NodeEvolverBatched
+    assert os.path.exists(FOREST_EXE), "build the Fortran binding first (__graft_entry__.build())"
+    physics = gp.physics_params()
+    t_z0 = float(wl.scale_factor_to_cosmic_time(1.0))
+    t_z5 = float(wl.scale_factor_to_cosmic_time(1.0 / 6.0))
+    cosmo, pch, walk = tr.cosmology(physics), tr.pch_params(2e10, t_z5), tr.walk_params(t_z0, merging_multiplier=0.2)
+    roots, seed = np.geomspace(1e11, 1e12, 6), 3
+    t = cooling_table
+    fin, fout = str(tmp_path / "forest.in"), str(tmp_path / "forest.out")
+    with open(fin, "wb") as f:
+        np.array([len(roots), t.n_t, t.n_z], dtype=np.int32).tofile(f)
+        np.array([seed], dtype=np.int64).tofile(f)
+        np.array([1e-6, 1e-3, t.ln_t_min, t.d_ln_t, t.ln_z_min, t.d_ln_z]).tofile(f)
+        np.ascontiguousarray(physics, dtype=np.float64).tofile(f)
+        np.ascontiguousarray(t.ln_lambda, dtype=np.float64).ravel().tofile(f)
+        roots.astype(np.float64).tofile(f)
+        np.array([t_z0]).tofile(f)
+        f.write(bytes(cosmo))
+        f.write(bytes(pch))
+        f.write(bytes(walk))
+        np.array([1e-4, 100.0, 0.25]).tofile(f)
+    print("running", FOREST_EXE, flush=True)
+    r = subprocess.run([FOREST_EXE, fin, fout], capture_output=True, text=True, timeout=120)
+    assert r.returncode == 0, r.stdout + r.stderr
+    raw = open(fout, "rb").read()
+    n = int(np.frombuffer(raw, dtype=np.int64, count=1)[0])
+    offsets = np.frombuffer(raw, dtype=np.int64, count=len(roots) + 1, offset=8)
+    o = 8 + 8 * (len(roots) + 1)
+    yf = np.frombuffer(raw, dtype=np.float64, count=30 * n, offset=o).reshape(30, n)
+    stf = np.frombuffer(raw, dtype=np.int32, count=n, offset=o + 240 * n)
+    rc = np.frombuffer(raw, dtype=np.int32, count=1, offset=o + 244 * n)[0]
+    assert rc == 0
+    forest = tr.Forest.build_pch(len(roots), roots, t_z0, cosmo, pch, seed=seed)
+    assert forest.n_nodes == n and np.array_equal(forest.tree_offset, offsets)
+    sched = tr.Schedule(forest, cosmo, walk)
+    ev = NodeEvolverBatched(n, physics, cooling_table, odeToleranceAbsolute=1e-6, odeToleranceRelative=1e-3)
+    y = torch.from_numpy(forest.initial_state(cosmo, walk)).cuda()
+    st = tr.evolve_forest(ev, sched, y).cpu().numpy()
+    print("nodes", n, "operations", sched.n_ops, flush=True)
+    assert np.array_equal(yf, y.cpu().numpy()) and np.array_equal(stf, st)
