@@ -131,7 +131,10 @@ def surrogate_leg(physics, table, tol_abs, tol_rel, args):
         ev.evolve_device(y, test.node_par, test.t0, test.t1, h)
         holder["rk"] = y
     t_rk, _ = _timed(run_rk)
-    gate = sg.GateConfig()
+    # The gate within the accuracy bound (median z=0 stellar-mass error of the
+    # 64-tree forest <= 1%) that runs fastest in the measured sweep: predicted
+    # error 0.03, baryon budget 1%, metal budget 5%, negatives down to -1 s.
+    gate = sg.GateConfig(error_max=0.03, mass_tolerance=1e-2, metal_tolerance=5e-2, negative_tolerance=1.0)
     se = sg.SurrogateEvolver(ev, mlp, norm, gate)
 
     def run_sur():

@@ -77,7 +77,7 @@ class GateConfig:
     mass_tolerance: float = 5.0e-3
     metal_tolerance: float = 2.0e-2
     bh_slack: float = 20.0
-    negative_tolerance: float = 1.0e-2   # small negatives (in units of the asinh scale) become zero
+    negative_tolerance: float = 0.1      # small negatives (in units of the asinh scale) become zero
 
     def c_struct(self):
         return _lib.Gate(math.log(self.error_max), self.mass_tolerance, self.metal_tolerance, self.bh_slack,
