@@ -508,9 +508,9 @@ def main():
                     help="run one elementwise copy of this many MiB first (reference kernel for HBM counters)")
     args = ap.parse_args()
     if args.gpus != 1:
-        print(f"bench.py measures one GPU: every box in this setup has one MI355X, and the batched evolver shards "
-              f"by whole node batches or forests (see DESIGN.md). --gpus {args.gpus} is not supported.",
-              file=sys.stderr)
+        print(f"bench.py measures one GPU. Across GPUs the engine would shard by whole node batches or whole "
+              f"forests with no exchange during evolution; that path is not built. --gpus {args.gpus} is not "
+              f"supported.", file=sys.stderr)
         sys.exit(2)
 
     import torch

@@ -7,7 +7,7 @@ contiguous), with ``Outputs/OutputN/mergerTreeIndex``,
 (merger_trees/outputter/standard.F90:235-392; the layout its tests read, e.g.
 testSuite/test-output-tree-contiguousity.py and test-reproducibility.py).
 This module writes the same group/dataset layout into a NumPy ``.npz``
-archive (no HDF5 library is available to this build): the keys are the HDF5
+archive (HDF5 is not a dependency of this package): the keys are the HDF5
 paths, ``read_output`` returns them as nested dictionaries, so an analysis
 that indexes ``f["Outputs/Output1/nodeData/diskMassStellar"]`` runs
 unchanged on the dictionary.

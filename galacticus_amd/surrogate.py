@@ -5,7 +5,7 @@
   training steps.
 * ``generate_pairs`` evolves node histories interval by interval with the
   FP64 batched integrator and keeps every (state, interval -> state) pair on
-  the GPU: the training data and the judge of the surrogate's error.
+  the GPU: the training data and the reference for the surrogate's error.
 * ``train_surrogate`` encodes the pairs, fits the normalisation, and trains.
 * ``SurrogateEvolver`` is the surrogate+fallback drop-in beside
   ``NodeEvolverBatched.evolve_device`` (same arguments, same statuses).
