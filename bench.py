@@ -588,7 +588,7 @@ def main():
     # evolver handles and streams, so one batch's tail (the few stiffest
     # nodes) overlaps the next batch's start.
     scales = [torch.empty_like(y0) for _ in range(n_streams)]
-    statuses = [torch.empty(n, dtype=torch.int32, device=dev) for _ in range(n_streams)]
+    statuses = [torch.zeros(n, dtype=torch.int32, device=dev) for _ in range(n_streams)]   # unused streams stay 0
     counters = [torch.zeros(3 * n, dtype=torch.int64, device=dev) for _ in range(n_streams)]
     torch.cuda.synchronize()
 
