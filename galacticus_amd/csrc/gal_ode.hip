@@ -605,8 +605,9 @@ int gal_ode_batch_create(int n_max, double tol_abs, double tol_rel, int flags, g
     gal_ode_batch_destroy(h);
     return GAL_E_NOMEM;
   }
-  static bool attrs_set = false;
-  if (!attrs_set) {
+  // Set on every handle creation (on the current device): no process-wide
+  // state, and cheap.
+  {
     static_assert(lds_bytes<64, true, true>() <= 163840, "lane-private LDS vectors must fit 160 KiB");
     static_assert(2 * lds_bytes<64, false, true>() <= 163840, "two workgroups per CU must fit 160 KiB");
     static_assert(4 * lds_bytes<64, true, false>() <= 163840, "four workgroups per CU must fit 160 KiB");
@@ -620,7 +621,6 @@ int gal_ode_batch_create(int n_max, double tol_abs, double tol_rel, int flags, g
       gal_ode_batch_destroy(h);
       return GAL_E_HIP;
     }
-    attrs_set = true;
   }
   *out = h;
   return GAL_OK;

@@ -953,13 +953,11 @@ int infer_layers(gal_mlp* m, int n, const uint16_t* x, float* out, hipStream_t s
 
 // Refresh the transposed weights from the current bf16 weights.
 int prepare_fused(gal_mlp* m, hipStream_t s) {
-  static bool attr_set = false;
-  if (!attr_set) {
-    static_assert(kFusedLds <= 163840, "fused forward LDS must fit a CU");
-    HIP_TRY(hipFuncSetAttribute((const void*)mlp_forward_fused, hipFuncAttributeMaxDynamicSharedMemorySize,
-                                (int)kFusedLds));
-    attr_set = true;
-  }
+  // the LDS limit of the kernel on the current device (a cheap host call; no
+  // process-wide state)
+  static_assert(kFusedLds <= 163840, "fused forward LDS must fit a CU");
+  HIP_TRY(hipFuncSetAttribute((const void*)mlp_forward_fused, hipFuncAttributeMaxDynamicSharedMemorySize,
+                              (int)kFusedLds));
   for (int l = 0; l < kLayers; ++l) {
     const int e = kFanIn[l] * kLdW[l];
     hipLaunchKernelGGL(transpose_weights_kernel, dim3((e + 255) / 256), dim3(256), 0, s, m->d_wbf[l], kFanIn[l],
