@@ -165,8 +165,9 @@ def surrogate_leg(physics, table, tol_abs, tol_rel, args):
     piped = pipelined_surrogate(ev, mlp, norm, gate, test, h0, launch, physics, table, tol_abs, tol_rel, args)
     log(f"surrogate: RK {rk_rate:.4g} node-Gyr/s, surrogate+fallback {sur_rate:.4g} ({sur_rate / rk_rate:.2f}x), "
         f"rejected {n_rej / n:.3f}, median M* error {float(ms_err.median()):.2e}")
-    log(f"surrogate pipelined over {piped['streams']} streams: RK {piped['rk_s_per_batch'] * 1e3:.2f} ms/batch, "
-        f"surrogate+fallback {piped['surrogate_s_per_batch'] * 1e3:.2f} ms/batch "
+    log(f"surrogate pipelined over {piped['streams']} streams: RK {piped['rk_s_per_batch'] * 1e3:.2f} ms/batch "
+        f"(launch {piped['rk_launch']}), surrogate+fallback {piped['surrogate_s_per_batch'] * 1e3:.2f} ms/batch "
+        f"(launch {piped['surrogate_launch']}) "
         f"({piped['rk_s_per_batch'] / piped['surrogate_s_per_batch']:.2f}x)")
     args.trained_surrogate = se
     return {
@@ -197,11 +198,18 @@ def dataclasses_asdict(x):
     return dataclasses.asdict(x)
 
 
+PIPELINE_SHAPES = ((64, 1), (64, 2), (64, 4))
+
+
 def pipelined_surrogate(ev, mlp, norm, gate, test, h0, launch, physics, table, tol_abs, tol_rel, args):
     """Throughput of both paths as bench's main leg measures the integrator:
     independent copies of the evaluation batch alternate over streams, each
     stream with its own integrator handle (and network handle), so one
-    batch's tail of stiff nodes overlaps the next batch's start."""
+    batch's tail of stiff nodes overlaps the next batch's start.  Each path
+    runs with the integrator launch shape that is fastest for it in this
+    overlapped regime (the fallback batch is a sparse subset, whose best
+    shape differs from the full batch's), chosen by one timed pass per shape
+    and then timed afresh."""
     import torch
     from galacticus_amd import surrogate as sg
     from galacticus_amd.evolver import NodeEvolverBatched
@@ -209,19 +217,16 @@ def pipelined_surrogate(ev, mlp, norm, gate, test, h0, launch, physics, table, t
     n = test.n
     evs = [ev] + [NodeEvolverBatched(n, physics, table, odeToleranceAbsolute=tol_abs, odeToleranceRelative=tol_rel)
                   for _ in range(n_streams - 1)]
-    for e in evs[1:]:
-        e.set_launch(*launch)
     params = mlp.get_params()
     mlps = [mlp] + [sg.SurrogateMLP(8192, params=params) for _ in range(n_streams - 1)]
     ses = [sg.SurrogateEvolver(e, m, norm, gate) for e, m in zip(evs, mlps)]
     streams = [torch.cuda.Stream() for _ in range(n_streams)]
-    status = [torch.empty(n, dtype=torch.int32, device=test.t0.device) for _ in range(n_streams)]
-    accepted = [torch.empty(n, dtype=torch.int32, device=test.t0.device) for _ in range(n_streams)]
+    status = [torch.zeros(n, dtype=torch.int32, device=test.t0.device) for _ in range(n_streams)]
+    accepted = [torch.ones(n, dtype=torch.int32, device=test.t0.device) for _ in range(n_streams)]
     out = {}
     for name in ("rk", "surrogate"):
         ys = [test.y0.clone() for _ in range(batches)]
         hs = [h0.clone() for _ in range(batches)]
-        n_rej = 0
 
         def launch_one(k):
             j = k % n_streams
@@ -230,24 +235,36 @@ def pipelined_surrogate(ev, mlp, norm, gate, test, h0, launch, physics, table, t
             else:
                 ses[j].evolve_device(ys[k], test.node_par, test.t0, test.t1, hs[k], status=status[j],
                                      accepted=accepted[j], stream=streams[j], count_rejected=False)
-        launch_one(0)                      # warm-up (allocations of first use)
-        torch.cuda.synchronize()
-        ys[0] = test.y0.clone()
-        hs[0] = h0.clone()
-        torch.cuda.synchronize()
-        t = time.perf_counter()
-        for k in range(batches):
-            launch_one(k)
-        torch.cuda.synchronize()
-        wall = time.perf_counter() - t
+
+        def timed_pass():
+            for k in range(batches):
+                ys[k].copy_(test.y0)
+                hs[k].copy_(h0)
+            torch.cuda.synchronize()
+            t = time.perf_counter()
+            for k in range(batches):
+                launch_one(k)
+            torch.cuda.synchronize()
+            return (time.perf_counter() - t) / batches
+
+        per_shape = {}
+        for shape in PIPELINE_SHAPES:
+            for e in evs:
+                e.set_launch(*shape)
+            launch_one(0)                  # warm-up (allocations of first use)
+            torch.cuda.synchronize()
+            per_shape[shape] = timed_pass()
+        best = min(per_shape, key=per_shape.get)
+        for e in evs:
+            e.set_launch(*best)
+        out[f"{name}_s_per_batch"] = timed_pass()
+        out[f"{name}_launch"] = list(best)
+        out[f"{name}_s_per_batch_by_launch"] = {f"{a},{b}": v for (a, b), v in per_shape.items()}
         if name == "surrogate":
-            n_rej = int((accepted[0] == 0).sum())
-        out[f"{name}_s_per_batch"] = wall / batches
-        if name == "surrogate":
-            out["rejected_fraction_last_batch"] = n_rej / n
-        ok = all(bool((st == 0).all()) for st in status)
-        out[f"{name}_statuses_ok"] = ok
+            out["rejected_fraction_last_batch"] = int((accepted[0] == 0).sum()) / n
+        out[f"{name}_statuses_ok"] = all(bool((st == 0).all()) for st in status)
         del ys, hs
+    ev.set_launch(*launch)
     out.update({"streams": n_streams, "batches": batches, "intervals_per_batch": n})
     for m in mlps[1:]:
         m.close()
