@@ -131,10 +131,12 @@ def surrogate_leg(physics, table, tol_abs, tol_rel, args):
         ev.evolve_device(y, test.node_par, test.t0, test.t1, h)
         holder["rk"] = y
     t_rk, _ = _timed(run_rk)
-    # The gate within the accuracy bound (median z=0 stellar-mass error of the
-    # 64-tree forest <= 1%) that runs fastest in the measured sweep: predicted
-    # error 0.03, baryon budget 1%, metal budget 5%, negatives down to -1 s.
-    gate = sg.GateConfig(error_max=0.03, mass_tolerance=1e-2, metal_tolerance=5e-2, negative_tolerance=1.0)
+    # Gate: predicted error 0.05, baryon budget 1.5%, metal budget 10%,
+    # negatives down to -1 s.  In the measured sweep it keeps the 64-tree
+    # forest's z=0 stellar masses within 0.5% (median) and 3% (max) of the
+    # integrator-only forest; a 2% baryon budget runs faster but lets single
+    # roots drift by 10%.
+    gate = sg.GateConfig(error_max=0.05, mass_tolerance=1.5e-2, metal_tolerance=1e-1, negative_tolerance=1.0)
     se = sg.SurrogateEvolver(ev, mlp, norm, gate)
 
     def run_sur():
@@ -209,7 +211,7 @@ def pipelined_surrogate(ev, mlp, norm, gate, test, h0, launch, physics, table, t
     runs with the integrator launch shape that is fastest for it in this
     overlapped regime (the fallback batch is a sparse subset, whose best
     shape differs from the full batch's), chosen by one timed pass per shape
-    and then timed afresh."""
+    and then timed afresh (median of three passes)."""
     import torch
     from galacticus_amd import surrogate as sg
     from galacticus_amd.evolver import NodeEvolverBatched
@@ -257,7 +259,7 @@ def pipelined_surrogate(ev, mlp, norm, gate, test, h0, launch, physics, table, t
         best = min(per_shape, key=per_shape.get)
         for e in evs:
             e.set_launch(*best)
-        out[f"{name}_s_per_batch"] = timed_pass()
+        out[f"{name}_s_per_batch"] = float(np.median([timed_pass() for _ in range(3)]))
         out[f"{name}_launch"] = list(best)
         out[f"{name}_s_per_batch_by_launch"] = {f"{a},{b}": v for (a, b), v in per_shape.items()}
         if name == "surrogate":
@@ -516,10 +518,10 @@ def main():
                     help="forests of a tree workload also run as this many concurrent forests on their own streams")
     ap.add_argument("--large-trees", type=int, default=256,
                     help="trees in the larger forest of the tree leg (0 skips it)")
-    ap.add_argument("--train-pairs", type=int, default=4000000)
-    ap.add_argument("--train-epochs", type=int, default=20)
+    ap.add_argument("--train-pairs", type=int, default=16000000)
+    ap.add_argument("--train-epochs", type=int, default=30)
     ap.add_argument("--eval-pairs", type=int, default=1000000)
-    ap.add_argument("--pipeline-batches", type=int, default=8,
+    ap.add_argument("--pipeline-batches", type=int, default=16,
                     help="evaluation batches per path in the pipelined surrogate comparison")
     ap.add_argument("--copy-calibration-mib", type=int, default=0,
                     help="run one elementwise copy of this many MiB first (reference kernel for HBM counters)")

@@ -659,22 +659,42 @@ __global__ __launch_bounds__(256) void apply_kernel(GateArgs a) {
       const float* o = a.out + (size_t)i * kOutLd;
       const float* nm = a.norm;
       bool ok = (double)o[GAL_MLP_HEAD] <= a.gate.log_error_max;
-      double du[kNV];
-      euler_baseline(a.P, a.D, a.tab, a.cd, a.mode, np, f, t0, dt, y0, du);
+      double rate[kNV];
+      rhs(a.P, a.D, a.tab, a.cd, a.mode, np, t0, y0, rate);
 #pragma unroll
       for (int v = 0; v < kNV; ++v) {
         const int role = var_role(v);
-        const double ov = fmin(fmax((double)o[v], -(double)kOutClip), (double)kOutClip);
-        const double d = du[v] + (double)nm[2 * kIn + v] + (double)nm[3 * kIn + v] * sinh(ov);
         if (role == R_CONST) {
           y1[v] = y0[v];
         } else if (role == R_SPIN) {
           y1[v] = y0[v] + (np.satellite ? 0.0 : np.spin_rate) * dt;
-        } else if (role == R_ENERGY) {
-          y1[v] = y0[v] + f.s_energy * sinh(d);
         } else {
+          // u1 = asinh(y0/s) + du + c with du the first-order increment of
+          // euler_baseline and c = m + sigma sinh(o) the network's residual.
+          // Written as u1 = asinh(Y/s) + C for the first-order end point Y,
+          // y1 = s sinh(u1) = Y cosh C + sqrt(s^2 + Y^2) sinh C: no asinh,
+          // and sinh C, cosh C from one expm1 (accurate for small C).
+          const double ov = fmin(fmax((double)o[v], -(double)kOutClip), (double)kOutClip);
+          double C = (double)nm[2 * kIn + v] + (double)nm[3 * kIn + v] * sinh(ov);
           const double s = var_scale(f, role);
-          y1[v] = s * sinh(asinh(y0[v] / s) + d);
+          const double step = rate[v] * dt;
+          double Y;
+          if (role == R_ENERGY) {
+            Y = step;                                  // energy: y1 = y0 + s sinh(asinh(f dt/s) + c)
+          } else if (rate[v] >= 0.0) {
+            Y = y0[v] + step;                          // linear growth
+          } else if (y0[v] > 0.0) {
+            Y = y0[v] * exp(step / y0[v]);             // decay on y/|f|
+          } else {
+            Y = y0[v];                                 // Euler step in u
+            C += step * rsqrt(s * s + y0[v] * y0[v]);
+          }
+          const double em = expm1(C);
+          const double e = 1.0 + em;
+          const double sinh_c = 0.5 * em * (1.0 + 1.0 / e);
+          const double cosh_c = 1.0 + 0.5 * em * em / e;
+          const double w = Y * cosh_c + sqrt(s * s + Y * Y) * sinh_c;
+          y1[v] = role == R_ENERGY ? y0[v] + w : w;
         }
         // a small negative value of a non-negative quantity is set to zero,
         // as the reference's post-step fixes do after an integrator step
