@@ -50,11 +50,11 @@ constexpr int64_t param_count() {
 constexpr int64_t kParams = param_count();
 
 // ---- bf16 ------------------------------------------------------------------
-__device__ __forceinline__ uint16_t f2bf(float f) {  // round to nearest even
-  uint32_t u = __float_as_uint(f);
-  if ((u & 0x7fffffffu) > 0x7f800000u) return (uint16_t)((u >> 16) | 0x40u);
-  u += 0x7fffu + ((u >> 16) & 1u);
-  return (uint16_t)(u >> 16);
+__device__ __forceinline__ uint16_t f2bf(float f) {  // round to nearest even; NaN stays quiet NaN
+  const uint32_t u = __float_as_uint(f);
+  const uint32_t rounded = (u + 0x7fffu + ((u >> 16) & 1u)) >> 16;
+  const uint32_t quiet = (u >> 16) | 0x40u;
+  return (uint16_t)(((u & 0x7fffffffu) > 0x7f800000u) ? quiet : rounded);   // a select, not a branch
 }
 __device__ __forceinline__ float bf2f(uint16_t b) { return __uint_as_float(((uint32_t)b) << 16); }
 __device__ __forceinline__ float silu(float z) { return z / (1.0f + __expf(-z)); }
