@@ -28,10 +28,13 @@ sys.path.insert(0, ROOT)
 
 # Independent batches overlap on separate streams only as far as the HIP
 # runtime gives those streams separate hardware queues; with its default of 4
-# queues per process, four streams measured 18.6 ms per step, with 8 queues
-# 10.0 ms.  Read when the HIP runtime starts, so it is set before torch is.
-if int(os.environ.get("GPU_MAX_HW_QUEUES", "4")) < 8:
-    os.environ["GPU_MAX_HW_QUEUES"] = "8"
+# queues per process, four streams measured 18.6 ms per step, with 8 or 16
+# queues 10.0 ms.  The later legs create streams of their own (surrogate,
+# concurrent forests): with 8 queues the four forest streams shared queues
+# (64 trees in 0.29 s), with 16 they did not (0.17 s).  Read when the HIP
+# runtime starts, so it is set before torch is.
+if int(os.environ.get("GPU_MAX_HW_QUEUES", "4")) < 16:
+    os.environ["GPU_MAX_HW_QUEUES"] = "16"
 
 METRIC = "node-Gyr evolved/sec: FP64 batched RK vs MLP+fallback at ≤1% M* error"
 FP64_PEAK_TFLOPS = 78.6
@@ -353,7 +356,7 @@ def chained_histories(ev, se, physics, args, n=20000):
             "rk_seconds": times["rk"], "surrogate_seconds": times["surrogate"]}
 
 
-def _forest_run(physics, table, tol_abs, tol_rel, n_trees, log_m, z_start, label, cpu_threads, cpu=True, split=1,
+def _forest_run(physics, table, tol_abs, tol_rel, n_trees, log_m, z_start, label, cpu_threads, cpu=True, split=(),
                 surrogate=None):
     """Build a PCH forest, record its walk, evolve it wave by wave on the GPU
     and (cpu) with the CPU oracle's tree runner over cpu_threads threads;
@@ -393,15 +396,18 @@ def _forest_run(physics, table, tol_abs, tol_rel, n_trees, log_m, z_start, label
     log(f"trees ({label}): {forest.n_nodes} nodes, {sched.n_ops} operations in {sched.n_waves} waves, "
         f"GPU {gpu_s:.3f} s ({gyr / gpu_s:.4g} node-Gyr/s)")
     del holder, ev
-    if split > 1:
-        # the same trees as `split` independent forests, evolved concurrently
-        # on their own streams: a wave of one no longer waits for the
-        # slowest interval of all the others
-        parts = forest.split(split)
+    # the same trees as F independent forests, evolved concurrently over S
+    # streams (forest k on stream k mod S): a wave of one no longer waits for
+    # the slowest interval of all the others.  Smaller forests have shorter
+    # waves, but when every wave fills the GPU, more launches in flight only
+    # interfere: each (F, S) in `split` is timed and the fastest is kept.
+    for n_forests, n_streams in split:
+        parts = forest.split(n_forests)
         scheds = [tr.Schedule(f, cosmo, wp) for f, _ in parts]
         evs = [NodeEvolverBatched(f.n_nodes, physics, table, odeToleranceAbsolute=tol_abs,
                                   odeToleranceRelative=tol_rel) for f, _ in parts]
-        streams = [torch.cuda.Stream() for _ in parts]
+        pool = [torch.cuda.Stream() for _ in range(min(n_streams, len(parts)))]
+        streams = [pool[k % len(pool)] for k in range(len(parts))]
         hold = {}
 
         def run_split():
@@ -411,12 +417,14 @@ def _forest_run(physics, table, tol_abs, tol_rel, n_trees, log_m, z_start, label
         split_s, _ = _timed(run_split)
         ys = np.concatenate([y.cpu().numpy() for y in hold["y"]], axis=1)
         same = bool(np.array_equal(ys, yg))
-        out["split"] = {"forests": len(parts), "waves_max": max(sc.n_waves for sc in scheds), "gpu_seconds": split_s,
-                        "node_gyr_per_s": gyr / split_s, "identical_to_single_forest": same}
-        log(f"trees ({label}): as {len(parts)} concurrent forests {split_s:.3f} s ({gyr / split_s:.4g} node-Gyr/s), "
-            f"identical states: {same}")
-        if split_s < gpu_s:
+        rec = {"forests": len(parts), "streams": len(pool), "waves_max": max(sc.n_waves for sc in scheds),
+               "gpu_seconds": split_s, "node_gyr_per_s": gyr / split_s, "identical_to_single_forest": same}
+        out.setdefault("split_by_forests", {})[f"{len(parts)}x{len(pool)}"] = rec
+        log(f"trees ({label}): as {len(parts)} concurrent forests on {len(pool)} streams {split_s:.3f} s "
+            f"({gyr / split_s:.4g} node-Gyr/s), identical states: {same}")
+        if same and split_s < out["gpu_seconds"]:
             out["gpu_seconds"], out["node_gyr_per_s"] = split_s, gyr / split_s
+            out["split"] = rec
         del hold, evs
     y_sur = None
     if surrogate is not None:
@@ -489,13 +497,13 @@ def trees_leg(physics, table, tol_abs, tol_rel, args):
     threads = min(16, os.cpu_count() or 1)
     small = _forest_run(physics, table, tol_abs, tol_rel, 64, (10.0, 12.0), 5.0,
                         "64 PCH trees, seed 0, roots log-uniform 1e10-1e12 Msun, resolution 1e9 Msun, z=5 -> 0",
-                        threads, cpu=not args.no_cpu, split=args.tree_streams)
+                        threads, cpu=not args.no_cpu, split=args.tree_splits)
     out = dict(small)
     if args.large_trees > 0:
         out["large"] = _forest_run(physics, table, tol_abs, tol_rel, args.large_trees, (11.0, 13.0), 8.0,
                                    f"{args.large_trees} PCH trees, seed 0, roots log-uniform 1e11-1e13 Msun, "
                                    f"resolution 1e9 Msun, z=8 -> 0", threads, cpu=not args.no_cpu,
-                                   split=args.tree_streams, surrogate=getattr(args, "trained_surrogate", None))
+                                   split=args.tree_splits, surrogate=getattr(args, "trained_surrogate", None))
     return out
 
 
@@ -514,8 +522,9 @@ def main():
     ap.add_argument("--cpu-seconds", type=float, default=15.0)
     ap.add_argument("--no-surrogate", action="store_true", help="skip the surrogate training/inference legs")
     ap.add_argument("--no-trees", action="store_true", help="skip the merger-tree leg")
-    ap.add_argument("--tree-streams", type=int, default=4,
-                    help="forests of a tree workload also run as this many concurrent forests on their own streams")
+    ap.add_argument("--tree-splits", default="2,4",
+                    help="comma-separated splits of a tree workload into concurrent forests, F (F forests on F "
+                         "streams) or FxS (F forests on S streams); the fastest is reported")
     ap.add_argument("--large-trees", type=int, default=256,
                     help="trees in the larger forest of the tree leg (0 skips it)")
     ap.add_argument("--train-pairs", type=int, default=16000000)
@@ -526,6 +535,8 @@ def main():
     ap.add_argument("--copy-calibration-mib", type=int, default=0,
                     help="run one elementwise copy of this many MiB first (reference kernel for HBM counters)")
     args = ap.parse_args()
+    args.tree_splits = tuple((int(k.split("x")[0]), int(k.split("x")[-1])) for k in str(args.tree_splits).split(",")
+                             if k.strip() and int(k.split("x")[0]) > 1)
     if args.gpus != 1:
         print(f"bench.py measures one GPU. Across GPUs the engine would shard by whole node batches or whole "
               f"forests with no exchange during evolution; that path is not built. --gpus {args.gpus} is not "

@@ -319,9 +319,12 @@ class Schedule:
 
 def evolve_forests(evolvers, schedules, ys, streams, major_ratio=0.25):
     """Evolve independent forests concurrently: forest k on evolver k and
-    stream k, each driven by its own host thread (the library call releases
-    the interpreter lock), so one forest's waves overlap the others'.
-    Returns the per-node status tensors."""
+    stream streams[k], each driven by its own host thread (the library call
+    releases the interpreter lock), so one forest's waves overlap the
+    others'.  Several forests may share a stream: their launches then
+    interleave in that stream's order (the forests touch disjoint state, so
+    the results do not depend on the interleaving).  Returns the per-node
+    status tensors."""
     import threading
     out = [None] * len(schedules)
     err = []

@@ -164,6 +164,14 @@ def test_concurrent_forests_equal_one_forest(torch_cuda, cooling_table):
     st_split = np.concatenate([s.cpu().numpy() for s in sts])
     assert np.array_equal(y_split, yd.cpu().numpy())
     assert np.array_equal(st_split, st)
+    # the same three forests with two of them sharing one stream: their
+    # launches interleave in that stream, the states do not change
+    shared = torch.cuda.Stream()
+    ys = [torch.from_numpy(np.ascontiguousarray(y0[:, n0:n0 + f.n_nodes])).cuda() for f, n0 in parts]
+    sts = tr.evolve_forests(evs, scheds, ys, [shared, torch.cuda.Stream(), shared])
+    torch.cuda.synchronize()
+    assert np.array_equal(np.concatenate([y.cpu().numpy() for y in ys], axis=1), yd.cpu().numpy())
+    assert np.array_equal(np.concatenate([s.cpu().numpy() for s in sts]), st)
 
 
 def test_surrogate_route_rejecting_everything_equals_plain_forest(torch_cuda, cooling_table):
