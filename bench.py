@@ -522,7 +522,7 @@ def main():
     ap.add_argument("--cpu-seconds", type=float, default=15.0)
     ap.add_argument("--no-surrogate", action="store_true", help="skip the surrogate training/inference legs")
     ap.add_argument("--no-trees", action="store_true", help="skip the merger-tree leg")
-    ap.add_argument("--tree-splits", default="2,4",
+    ap.add_argument("--tree-splits", default="4,8",
                     help="comma-separated splits of a tree workload into concurrent forests, F (F forests on F "
                          "streams) or FxS (F forests on S streams); the fastest is reported")
     ap.add_argument("--large-trees", type=int, default=256,
