@@ -72,6 +72,8 @@ def lib():
         L.orc_test_ode_solve.argtypes = [ctypes.c_int, ctypes.c_double, ctypes.c_double, dp, ctypes.c_double,
                                          ctypes.c_double, dp, lp]
         L.orc_test_ode_solve.restype = ctypes.c_int
+        L.orc_test_rkck_step.argtypes = [ctypes.c_int, ctypes.c_double, ctypes.c_double, dp, dp]
+        L.orc_test_rkck_step.restype = ctypes.c_int
         L.orc_tree_run.argtypes = [ctypes.c_int64, ip, ip, ip, dp, dp, dp, ctypes.c_int, ip, ctypes.c_int, dp, dp, ip,
                                    dp, cp, ctypes.c_double, ctypes.c_double, ctypes.c_int, ctypes.c_double,
                                    ctypes.c_double, ctypes.c_int]
@@ -185,6 +187,14 @@ def test_ode_solve(which, x0, x1, y0, tol_abs, tol_rel):
     cnt = np.zeros(3, dtype=np.int64)
     st = lib().orc_test_ode_solve(which, x0, x1, _dp(y), tol_abs, tol_rel, _dp(h), _lp(cnt))
     return st, y, float(h[0]), cnt
+
+
+def test_rkck_step(which, t, h, y0):
+    """One uncontrolled Cash-Karp step of a test system: (status, y, yerr)."""
+    y = np.array(y0, dtype=np.float64, copy=True)
+    yerr = np.zeros_like(y)
+    st = lib().orc_test_rkck_step(which, t, h, _dp(y), _dp(yerr))
+    return st, y, yerr
 
 
 def tree_run(ops, y0, h0, tree_of_node, n_trees, params, table, tol_abs, tol_rel, flags, major_ratio=0.25,

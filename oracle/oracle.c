@@ -1277,8 +1277,22 @@ static int test_rhs(void *ctx, double t, const double *y, double *dydt)
   test_ctx *c = (test_ctx *)ctx;
   c->n_rhs++;
   if (c->which == 1) dydt[0] = sin(t);
+  else if (c->which == 3) dydt[0] = y[0];
   else { dydt[0] = y[1]; dydt[1] = -1.0 * y[0]; }
   return ST_SUCCESS;
+}
+
+int orc_test_rkck_step(int which, double t, double h, double *y, double *yerr)
+{
+  test_ctx c = { which, 0 };
+  orc_driver d;
+  memset(&d, 0, sizeof(d));
+  d.dim = (which == 2) ? 2 : 1;
+  d.f = test_rhs; d.ctx = &c;
+  double dydt_in[NV], dydt_out[NV];
+  int st = test_rhs(&c, t, y, dydt_in);
+  if (st) return st;
+  return rkck_apply(&d, t, h, y, yerr, dydt_in, dydt_out);
 }
 
 int orc_test_ode_solve(int which, double x0, double x1, double *y, double tol_abs, double tol_rel,

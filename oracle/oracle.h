@@ -94,6 +94,11 @@ int orc_test_ode_solve(int which, double x0, double x1, double *y,
                        double tol_abs, double tol_rel, double *h_out,
                        int64_t counters[3]);
 
+/* One Cash-Karp step (no step-size control) of a built-in test system from
+   (t, y): y is replaced by the fifth-order solution, yerr receives the
+   embedded error estimate.  which: 1, 2 as above; 3 = y'=y.              */
+int orc_test_rkck_step(int which, double t, double h, double *y, double *yerr);
+
 #ifdef __cplusplus
 }
 #endif

@@ -76,6 +76,34 @@ def test_cash_karp_fifth_order_convergence():
     assert errs[0] < 1e-6 and errs[2] < 1e-8
 
 
+def test_cash_karp_single_step_orders():
+    """One uncontrolled step of y'=y from y=1 is the method's stability
+    polynomial R(h) = T5(h) + c h^6, T5 the degree-5 Taylor polynomial of
+    exp: fifth order fixes every coefficient through h^5 whatever the tableau,
+    so halving h divides R - T5 by 64 exactly.  The error estimate (fifth
+    minus embedded fourth order) starts at h^5: halving h divides it by about
+    32.  Both follow from the published orders, not from the coefficients,
+    so a mistyped coefficient or a dropped stage breaks them."""
+    def t5(h):
+        return sum(h ** k / math.factorial(k) for k in range(6))
+    dev, err = [], []
+    for h in (0.2, 0.1, 0.05, 0.025):
+        st, y, yerr = oracle.test_rkck_step(3, 0.0, h, [1.0])
+        assert st == 0
+        dev.append(y[0] - t5(h))
+        err.append(yerr[0])
+    for a, b in zip(dev[:-1], dev[1:]):
+        assert a / b == pytest.approx(64.0, rel=1e-3), dev
+    for a, b in zip(err[1:-1], err[2:]):
+        assert a / b == pytest.approx(32.0, rel=0.05), err
+    assert abs(dev[0]) > 1e-9 and abs(err[0]) > 1e-8
+    # the harmonic oscillator: one step stays within O(h^6) of the exact rotation
+    for h in (0.1, 0.05):
+        st, y, _ = oracle.test_rkck_step(2, 0.0, h, [1.0, 0.0])
+        assert st == 0
+        assert abs(y[0] - math.cos(h)) < 1e-3 * h ** 6 and abs(y[1] + math.sin(h)) < 1e-3 * h ** 6
+
+
 # ------------------------------------------------------------------ a5 -----
 def _hadj(y, yerr, scale, h, nonneg=None, eps_abs=1e-6, eps_rel=1e-3):
     y = np.asarray(y, float)
