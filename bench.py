@@ -100,6 +100,24 @@ def _timed(fn, reps=2):
     return best, out
 
 
+def parse_tree_splits(text):
+    """'4,8,4x2' -> ((4, 4), (8, 8), (4, 2)): F forests on S streams (S = F
+    when omitted); splits into fewer than two forests are dropped."""
+    out = []
+    for item in str(text).split(","):
+        item = item.strip()
+        if not item:
+            continue
+        f, _, st = item.partition("x")
+        n_forests = int(f)
+        n_streams = int(st) if st else n_forests
+        if n_streams < 1:
+            raise ValueError(f"tree split {item!r}: at least one stream")
+        if n_forests > 1:
+            out.append((n_forests, n_streams))
+    return tuple(out)
+
+
 def surrogate_leg(physics, table, tol_abs, tol_rel, args):
     """Surrogate training on integrator trajectories of merger trees drawn as
     the 64-tree workload (roots log-uniform 1e10-1e12 Msun, resolution 1e9
@@ -562,8 +580,7 @@ def main():
     ap.add_argument("--copy-calibration-mib", type=int, default=0,
                     help="run one elementwise copy of this many MiB first (reference kernel for HBM counters)")
     args = ap.parse_args()
-    args.tree_splits = tuple((int(k.split("x")[0]), int(k.split("x")[-1])) for k in str(args.tree_splits).split(",")
-                             if k.strip() and int(k.split("x")[0]) > 1)
+    args.tree_splits = parse_tree_splits(args.tree_splits)
     if args.gpus != 1:
         print(f"bench.py measures one GPU. Across GPUs the engine would shard by whole node batches or whole "
               f"forests with no exchange during evolution; that path is not built. --gpus {args.gpus} is not "
