@@ -274,16 +274,22 @@ def test_rejected_nodes_get_the_integrator_result(torch_cuda, cooling_table):
     assert torch.equal(y, y_ref) and torch.equal(h, h_ref) and torch.equal(st, st_ref)
 
 
-def test_accepted_nodes_get_the_decoded_network_output(torch_cuda, cooling_table):
+@pytest.mark.parametrize("centre,spread", [(0.01, 1.0e-3), (-0.05, 0.01), (1.5, 0.1)],
+                         ids=["small-growth", "shrink", "large-growth"])
+def test_accepted_nodes_get_the_decoded_network_output(torch_cuda, cooling_table, centre, spread):
     torch = torch_cuda
     from galacticus_amd.surrogate import SurrogateEvolver, GateConfig, PairSet, encode
     p, b, t1, ev, mlp, norm, d = _surrogate_setup(torch, cooling_table, n=600, seed=8)
-    # An untrained network's outputs become a small positive offset on top of
-    # the first-order step (every decoded state then stays non-negative); the
-    # head threshold at the median splits the nodes into accepted and rejected.
+    # An untrained network's outputs become an offset of about `centre` (in
+    # asinh units of each variable) on top of the first-order step: small
+    # growth keeps every decoded state non-negative, a small negative offset
+    # shrinks the reservoirs and takes empty ones to just below zero (within
+    # the clamp to zero), a large one multiplies them.  The head threshold at the median splits the nodes
+    # into accepted and rejected.  The device decodes in closed form
+    # (Y cosh C + sqrt(s^2 + Y^2) sinh C); the oracle decodes explicitly.
     norm = norm.clone()
-    norm[2, :30] = 0.01
-    norm[3, :30] = 1.0e-3
+    norm[2, :30] = centre
+    norm[3, :30] = spread
     pairs = PairSet(d(b.y), None, d(b.node_par), d(b.t_start), d(t1))
     x, _, _ = encode(ev, pairs, norm=norm, with_target=False)
     out = mlp.forward(x).cpu().numpy().astype(np.float64)
@@ -318,7 +324,7 @@ def test_accepted_nodes_get_the_decoded_network_output(torch_cuda, cooling_table
         else:
             assert not ok, i
             assert np.array_equal(yh[:, i], y_rk[:, i]), i
-    print("accepted", n_acc, "of", b.n, flush=True)
+    print("centre", centre, "accepted", n_acc, "of", b.n, flush=True)
     assert 0 < n_acc < b.n
 
 
