@@ -210,9 +210,10 @@ def surrogate_leg(physics, table, tol_abs, tol_rel, args):
     piped = pipelined_surrogate(ev, mlp, norm, gate, test, h0, launch, physics, table, tol_abs, tol_rel, args)
     log(f"surrogate: RK {rk_rate:.4g} node-Gyr/s, surrogate+fallback {sur_rate:.4g} ({sur_rate / rk_rate:.2f}x), "
         f"rejected {n_rej / n:.3f}, median M* error {float(ms_err.median()):.2e}")
-    log(f"surrogate pipelined over {piped['streams']} streams: RK {piped['rk_s_per_batch'] * 1e3:.2f} ms/batch "
-        f"(launch {piped['rk_launch']}), surrogate+fallback {piped['surrogate_s_per_batch'] * 1e3:.2f} ms/batch "
-        f"(launch {piped['surrogate_launch']}) "
+    log(f"surrogate pipelined: RK {piped['rk_s_per_batch'] * 1e3:.2f} ms/batch "
+        f"({piped['rk_streams']} streams, launch {piped['rk_launch']}), surrogate+fallback "
+        f"{piped['surrogate_s_per_batch'] * 1e3:.2f} ms/batch ({piped['surrogate_streams']} streams, "
+        f"launch {piped['surrogate_launch']}) "
         f"({piped['rk_s_per_batch'] / piped['surrogate_s_per_batch']:.2f}x)")
     args.trained_surrogate = se
     return {
@@ -253,14 +254,16 @@ def pipelined_surrogate(ev, mlp, norm, gate, test, h0, launch, physics, table, t
     independent copies of the evaluation batch alternate over streams, each
     stream with its own integrator handle (and network handle), so one
     batch's tail of stiff nodes overlaps the next batch's start.  Each path
-    runs with the integrator launch shape that is fastest for it in this
-    overlapped regime (the fallback batch is a sparse subset, whose best
-    shape differs from the full batch's), chosen by one timed pass per shape
-    and then timed afresh (median of three passes)."""
+    runs with the number of streams and the integrator launch shape that are
+    fastest for it in this overlapped regime (the fallback batch is a sparse
+    subset of stiff intervals, whose best configuration differs from the full
+    batch's): every combination gets one timed pass, the fastest is timed
+    afresh (median of three passes)."""
     import torch
     from galacticus_amd import surrogate as sg
     from galacticus_amd.evolver import NodeEvolverBatched
-    n_streams, batches = max(1, args.streams), args.pipeline_batches
+    counts = sorted({max(1, c) for c in args.pipeline_streams})
+    n_streams, batches = max(counts), args.pipeline_batches
     n = test.n
     evs = [ev] + [NodeEvolverBatched(n, physics, table, odeToleranceAbsolute=tol_abs, odeToleranceRelative=tol_rel)
                   for _ in range(n_streams - 1)]
@@ -275,44 +278,47 @@ def pipelined_surrogate(ev, mlp, norm, gate, test, h0, launch, physics, table, t
         ys = [test.y0.clone() for _ in range(batches)]
         hs = [h0.clone() for _ in range(batches)]
 
-        def launch_one(k):
-            j = k % n_streams
+        def launch_one(k, used):
+            j = k % used
             if name == "rk":
                 evs[j].evolve_device(ys[k], test.node_par, test.t0, test.t1, hs[k], status=status[j], stream=streams[j])
             else:
                 ses[j].evolve_device(ys[k], test.node_par, test.t0, test.t1, hs[k], status=status[j],
                                      accepted=accepted[j], stream=streams[j], count_rejected=False)
 
-        def timed_pass():
+        def timed_pass(used):
             for k in range(batches):
                 ys[k].copy_(test.y0)
                 hs[k].copy_(h0)
             torch.cuda.synchronize()
             t = time.perf_counter()
             for k in range(batches):
-                launch_one(k)
+                launch_one(k, used)
             torch.cuda.synchronize()
             return (time.perf_counter() - t) / batches
 
-        per_shape = {}
-        for shape in PIPELINE_SHAPES:
-            for e in evs:
-                e.set_launch(*shape)
-            launch_one(0)                  # warm-up (allocations of first use)
-            torch.cuda.synchronize()
-            per_shape[shape] = timed_pass()
-        best = min(per_shape, key=per_shape.get)
+        per_config = {}
+        for used in counts:
+            for shape in PIPELINE_SHAPES:
+                for e in evs:
+                    e.set_launch(*shape)
+                for j in range(used):      # warm-up (allocations of first use)
+                    launch_one(j % batches, used)
+                torch.cuda.synchronize()
+                per_config[(used, shape)] = timed_pass(used)
+        used, best = min(per_config, key=per_config.get)
         for e in evs:
             e.set_launch(*best)
-        out[f"{name}_s_per_batch"] = float(np.median([timed_pass() for _ in range(3)]))
+        out[f"{name}_s_per_batch"] = float(np.median([timed_pass(used) for _ in range(3)]))
         out[f"{name}_launch"] = list(best)
-        out[f"{name}_s_per_batch_by_launch"] = {f"{a},{b}": v for (a, b), v in per_shape.items()}
+        out[f"{name}_streams"] = used
+        out[f"{name}_s_per_batch_by_config"] = {f"{u} streams, {a},{b}": v for (u, (a, b)), v in per_config.items()}
         if name == "surrogate":
             out["rejected_fraction_last_batch"] = int((accepted[0] == 0).sum()) / n
         out[f"{name}_statuses_ok"] = all(bool((st == 0).all()) for st in status)
         del ys, hs
     ev.set_launch(*launch)
-    out.update({"streams": n_streams, "batches": batches, "intervals_per_batch": n})
+    out.update({"stream_counts": counts, "batches": batches, "intervals_per_batch": n})
     for m in mlps[1:]:
         m.close()
     return out
@@ -575,12 +581,15 @@ def main():
                     help="networks trained from different seeds; the one rejecting fewest validation intervals is kept")
     ap.add_argument("--val-pairs", type=int, default=262144)
     ap.add_argument("--eval-pairs", type=int, default=1000000)
+    ap.add_argument("--pipeline-streams", default="4,8",
+                    help="stream counts tried for each path of the pipelined surrogate comparison")
     ap.add_argument("--pipeline-batches", type=int, default=16,
                     help="evaluation batches per path in the pipelined surrogate comparison")
     ap.add_argument("--copy-calibration-mib", type=int, default=0,
                     help="run one elementwise copy of this many MiB first (reference kernel for HBM counters)")
     args = ap.parse_args()
     args.tree_splits = parse_tree_splits(args.tree_splits)
+    args.pipeline_streams = tuple(int(k) for k in str(args.pipeline_streams).split(",") if k.strip())
     if args.gpus != 1:
         print(f"bench.py measures one GPU. Across GPUs the engine would shard by whole node batches or whole "
               f"forests with no exchange during evolution; that path is not built. --gpus {args.gpus} is not "
