@@ -139,12 +139,12 @@ def surrogate_leg(physics, table, tol_abs, tol_rel, args):
     log(f"surrogate: {pairs.n} training pairs from tree trajectories in {gen_s:.2f} s")
     # Gate: predicted error 0.05, baryon budget 1.5%, metal budget 10%,
     # negatives down to -1 s.  In the measured sweep it keeps the 64-tree
-    # forest's z=0 stellar masses within 0.5% (median) and 3% (max) of the
-    # integrator-only forest; a 2% baryon budget runs faster but lets single
-    # roots drift by 10%.
+    # forest's z=0 stellar masses within about 0.5% (median) and 4% (max) of
+    # the integrator-only forest; a 2% baryon budget runs faster but lets
+    # single roots drift by 10%.
     gate = sg.GateConfig(error_max=0.05, mass_tolerance=1.5e-2, metal_tolerance=1e-1, negative_tolerance=1.0)
     # Model selection: networks trained from different initialisations and
-    # shuffles reject different fractions of intervals (9.5-12% in repeated
+    # shuffles reject different fractions of intervals (8-12% in repeated
     # trainings); each is scored on validation intervals of their own forests
     # (seeds 300, ...) and the one that rejects fewest is kept.  The
     # evaluation intervals below (seeds 200, ...) take no part in the choice.
