@@ -545,6 +545,15 @@ static void derive_constants(gal_ode_batch* h) {
                                           (D.cs2_sph * std::sqrt(D.cs2_sph))
                                     : 0.0;
   D.jeans_coeff = D.cs2_sph > 0.0 ? kGravity / D.cs2_sph : 0.0;
+  D.h0_gyr = h0 / kMpcPerKmsToGyr;
+  D.inv_one_minus_j_loss = 1.0 / (1.0 - p[GAL_P_FRACTION_LOSS_ANGULAR_MOMENTUM]);
+  // Ciotti & Ostriker wind coupling (black_holes/winds/Ciotti2009.F90:156-175):
+  // P_ISM / P_wind = 3 (3/2) k T_ISM v_wind Gyr / (m_H c^2 Mpc) * M_gas / (r eff Mdot)
+  {
+    constexpr double kVelocityWind = 1.0e4, kTemperatureISM = 1.0e4;  // km/s, K
+    D.wind_pressure_ratio = 3.0 * 1.5 * kBoltzmann * kTemperatureISM * kGigaYear * (kVelocityWind * kKilo) /
+                            (kMassHydrogenAtom * kSpeedLight * kSpeedLight * kMegaParsec);
+  }
 }
 
 extern "C" {
@@ -658,7 +667,9 @@ int gal_ode_batch_set_physics(gal_ode_batch* h, const double* params, int n_para
   if (!(params[GAL_P_OMEGA_MATTER] > 0.0) || !(params[GAL_P_OMEGA_DARK_ENERGY] > 0.0) ||
       !(params[GAL_P_HUBBLE_CONSTANT] > 0.0) || !(params[GAL_P_FB_DISK_VELOCITY] > 0.0) ||
       !(params[GAL_P_FB_SPHEROID_VELOCITY] > 0.0) || !(params[GAL_P_FB_DISK_TIMESCALE_MINIMUM] > 0.0) ||
-      !(params[GAL_P_FB_SPHEROID_TIMESCALE_MINIMUM] > 0.0) || !(params[GAL_P_CORE_RADIUS_OVER_VIRIAL] > 0.0))
+      !(params[GAL_P_FB_SPHEROID_TIMESCALE_MINIMUM] > 0.0) || !(params[GAL_P_CORE_RADIUS_OVER_VIRIAL] > 0.0) ||
+      !(params[GAL_P_FRACTION_LOSS_ANGULAR_MOMENTUM] < 1.0) || !(params[GAL_P_OPERATORS] >= 0.0) ||
+      !(params[GAL_P_OPERATORS] <= 15.0))
     return GAL_E_INVALID;
   std::memcpy(h->params, params, sizeof(double) * GAL_NPARAM);
   derive_constants(h);

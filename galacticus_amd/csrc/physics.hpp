@@ -31,6 +31,7 @@
 namespace gal {
 
 constexpr int kNV = GAL_NVAR;
+constexpr double kDoubleMax = 1.7976931348623157e308;   // huge(0d0)
 constexpr int kNoHalfPower = -100000;
 
 // Constants derived from the physics parameters once per set_physics call.
@@ -61,6 +62,9 @@ struct DerivedConstants {
   double inv_cs2_sph;                            // 1/cs^2 at the spheroid accretion temperature
   double bhl_sph_coeff;                          // enhancement * 4 pi G^2 (unit) / cs^3, spheroid mode
   double jeans_coeff;                            // G / cs^2 at the spheroid accretion temperature
+  double h0_gyr;                                 // H0 [1/Gyr]
+  double inv_one_minus_j_loss;                   // 1 / (1 - fractionLossAngularMomentum)
+  double wind_pressure_ratio;                    // P_ISM / P_wind = this * M_gas / (r eff Mdot)
 };
 
 // Reciprocal to full double precision: hardware estimate plus two
@@ -83,7 +87,7 @@ struct CoolingDesc {
 };
 
 struct NodePar {
-  double t_ref, m_ref, m_rate, spin_rate;
+  double t_ref, m_ref, m_rate, spin_rate, ram_force, v_host;
   bool satellite;
 };
 
@@ -96,6 +100,8 @@ __device__ __forceinline__ NodePar load_node_par(const double* __restrict__ np, 
   p.m_rate = np[GAL_NP_MASS_RATE * (size_t)ld + i];
   p.spin_rate = np[GAL_NP_SPIN_RATE * (size_t)ld + i];
   p.satellite = np[GAL_NP_IS_SATELLITE * (size_t)ld + i] != 0.0;
+  p.ram_force = np[GAL_NP_RAM_PRESSURE_FORCE * (size_t)ld + i];
+  p.v_host = np[GAL_NP_HOST_VELOCITY_VIRIAL * (size_t)ld + i];
   return p;
 }
 
@@ -177,6 +183,7 @@ __device__ __forceinline__ double hernquist_density(double mass, double scale, d
 // Halo quantities shared by the RHS, the scale rules and the post-step.
 struct Halo {
   double mass, a3, r_vir, inv_r_vir, v_vir, inv_v, t_dyn, inv_t_dyn, lambda, r_gal;
+  double r_vir_rate;   // darkMatterHaloScale%radiusVirialGrowthRate
 };
 
 __device__ __forceinline__ Halo halo(const double* __restrict__ P, const DerivedConstants& D, const NodePar& np,
@@ -184,8 +191,12 @@ __device__ __forceinline__ Halo halo(const double* __restrict__ P, const Derived
   Halo h;
   const double mrate = np.satellite ? 0.0 : np.m_rate;
   h.mass = np.m_ref + mrate * (t - np.t_ref);
+  // The virial scales use the mean density at the time the node was last
+  // isolated (dark_matter_halos/scales/virial_density_contrast.F90:356-382):
+  // a satellite's reference time is that time.
+  const double t_iso = np.satellite ? np.t_ref : t;
   // a^3 = (Omega_m/Omega_L) sinh^2(x), sinh(x) = (e^x - e^-x)/2
-  const double ex = exp(D.sinh_arg * t);
+  const double ex = exp(D.sinh_arg * t_iso);
   const double s = 0.5 * (ex - rcp(ex));
   h.a3 = D.a3_norm * s * s;
   const double om_a3 = D.om * rcp(h.a3);
@@ -204,6 +215,13 @@ __device__ __forceinline__ Halo halo(const double* __restrict__ P, const Derived
   const double jscale = h.mass * h.v_vir * h.r_vir;
   h.lambda = jscale > 0.0 ? spin_j * rcp(jscale) : 0.0;
   h.r_gal = P[GAL_P_RADIUS_FACTOR] * h.r_vir * h.lambda;  // fixed.F90:318-325
+  // radiusVirialGrowthRate (virial_density_contrast.F90:338-354):
+  // (r_vir/3)(Mdot/M - dln(Delta_c E^2)/dt), dln E^2/dt = -3 H Omega_m(a),
+  // dOmega_m(a)/dt = -3 H Omega_m(a)(1 - Omega_m(a)); zero for satellites.
+  const double hub = D.h0_gyr * sqrt(e2);
+  const double dx = -3.0 * hub * om_a * (1.0 - om_a);
+  const double dln = (82.0 - 78.0 * x) * dx * rcp(delta_c) - 3.0 * hub * om_a;
+  h.r_vir_rate = np.satellite ? 0.0 : h.r_vir * (1.0 / 3.0) * (mrate * rcp(h.mass) - dln);
   return h;
 }
 
@@ -261,6 +279,11 @@ __device__ __forceinline__ void rhs(const double* __restrict__ P, const DerivedC
 
   // spin%angularMomentum growth (halo_angular_momentum_interpolate.F90)
   f[GAL_V_SPIN_ANGULAR_MOMENTUM] += np.satellite ? 0.0 : np.spin_rate;
+  const int ops = box ? 0 : (int)P[GAL_P_OPERATORS];
+  // satelliteMassLoss (satellite/mass_loss/_class.F90:176-200): an isolated
+  // halo's bound mass follows its basic mass; darkMatterHaloMassLossRate is
+  // the default zero class for satellites.
+  f[GAL_V_SATELLITE_BOUND_MASS] += ((ops & GAL_OPERATOR_SATELLITE_MASS_LOSS) && !np.satellite) ? np.m_rate : 0.0;
 
   const double R = P[GAL_P_RECYCLED_FRACTION], yld = P[GAL_P_METAL_YIELD];
   // ---- starFormationDisks (star_formation/disks.F90:200-325) -----------
@@ -287,12 +310,13 @@ __device__ __forceinline__ void rhs(const double* __restrict__ P, const DerivedC
     f[GAL_V_DISK_MASS_STELLAR_FORMED] += sfr_d;
   }
   // ---- starFormationSpheroids (star_formation/spheroids.F90:152) --------
-  const bool sph_ok = !box && !(j_s < 0.0 || r_gal < 0.0 || mg_s < 0.0);
+  const bool sph_ok = !(j_s < 0.0 || r_gal < 0.0 || mg_s < 0.0);
   double tau_s;
   {
     const double eff = P[GAL_P_SF_SPHEROID_EFFICIENCY];
     const double pw = pow_param(x_sf, sqrt_x_sf, ln_v - D.ln_200, D.n2_sf_sph, P[GAL_P_SF_SPHEROID_EXPONENT_VELOCITY]);
     tau_s = (v_gal <= 0.0 || eff == 0.0) ? 0.0 : fmax(tdyn_gal * pw * D.inv_eff_sf_sph, P[GAL_P_SF_SPHEROID_TIMESCALE_MINIMUM]);
+    if (box) tau_s = P[GAL_P_BOX_TIMESCALE];  // starFormationTimescale fixed
   }
   const double sfr_s = (sph_ok && tau_s > 0.0) ? mg_s * rcp(tau_s) : 0.0;
   {
@@ -326,30 +350,72 @@ __device__ __forceinline__ void rhs(const double* __restrict__ P, const DerivedC
     double o = pow_param(w, D.sqrt_v_fb_sph * inv_sqrt_v, D.ln_v_fb_sph - ln_v, D.n2_fb_sph,
                          P[GAL_P_FB_SPHEROID_EXPONENT]) * sfr_s;
     o = v_gal <= 0.0 ? 0.0 : o;
+    if (box) o = P[GAL_P_BOX_OUTFLOW_FRACTION_SPHEROID] * sfr_s;  // stellarFeedbackOutflows fixed
     const double omax = fmax(mg_s * inv_tdyn_fb * D.inv_tmin_fb_sph, 0.0);
     o = o > omax ? omax : o;
     out_s = (sph_ok && o > 0.0) ? o : 0.0;
   }
+  // Outflowing gas joins the outflowed reservoir (hot_halo/standard/_class.F90:598-720):
+  // the stripped fraction is zero for centrals and for satellites within the
+  // virial radius (outflow_stripping/standard.F90:133-173), and the angular
+  // momentum is raised by 1/(1 - fractionLossAngularMomentum).
   {
     const double rj = j_d * (out_d * inv_mt_d);
     const double rz = (zg_d * inv_mg_d) * out_d;
     f[GAL_V_HOT_OUTFLOWED_MASS] += out_d;
     f[GAL_V_DISK_MASS_GAS] -= out_d;
-    f[GAL_V_HOT_OUTFLOWED_ANGULAR_MOMENTUM] += rj;
+    f[GAL_V_HOT_OUTFLOWED_ANGULAR_MOMENTUM] += rj * D.inv_one_minus_j_loss;
     f[GAL_V_DISK_ANGULAR_MOMENTUM] -= rj;
     f[GAL_V_HOT_OUTFLOWED_ABUNDANCES] += rz;
     f[GAL_V_DISK_ABUNDANCES_GAS] -= rz;
   }
-  if (box) return;
   {
     const double rj = j_s * (out_s * inv_mt_s);
     const double rz = (zg_s * inv_mg_s) * out_s;
     f[GAL_V_HOT_OUTFLOWED_MASS] += out_s;
     f[GAL_V_SPHEROID_MASS_GAS] -= out_s;
-    f[GAL_V_HOT_OUTFLOWED_ANGULAR_MOMENTUM] += rj;
+    f[GAL_V_HOT_OUTFLOWED_ANGULAR_MOMENTUM] += rj * D.inv_one_minus_j_loss;
     f[GAL_V_SPHEROID_ANGULAR_MOMENTUM] -= rj;
     f[GAL_V_HOT_OUTFLOWED_ABUNDANCES] += rz;
     f[GAL_V_SPHEROID_ABUNDANCES_GAS] -= rz;
+  }
+  if (box) return;
+  // ---- barInstability (bar_instability.F90:145-248) with efstathiou1982
+  // (galactic_dynamics/bar_instability/Efstathiou1982.F90:153-258): unstable
+  // when eps = 1.18 V / sqrt(G M_disk / r_disk) is below the threshold
+  // interpolated in gas fraction; material moves to the spheroid on
+  // t_dyn ((eps_c - eps_iso)/(eps_c - eps))^2.  A disk of zero mass gives a
+  // NaN gas fraction and, as in the reference, no instability.
+  {
+    constexpr double kEpsIso = 0.6221297315, kBoost = 1.1800237580, kDimMax = 1.0e10;
+    const double fgas = mg_d * rcp(mt_d);
+    const double thr = P[GAL_P_BAR_THRESHOLD_STELLAR] * (1.0 - fgas) + P[GAL_P_BAR_THRESHOLD_GASEOUS] * fgas;
+    const double v_self2 = kGravity * mt_d * rcp(r_gal);
+    double est = fmax(kEpsIso, kBoost * v_gal * rsqrt(v_self2));
+    est = (mt_d >= 0.0 && v_self2 > 0.0) ? est : HUGE_VAL;
+    const bool unstable = (ops & GAL_OPERATOR_BAR_INSTABILITY) && j_d > 0.0 && r_gal > 0.0 && v_gal > 0.0 &&
+                          mg_d >= 0.0 && est < thr;
+    constexpr double kC = kSpeedLight / kKilo;
+    const double t_dyn = v_gal < kC ? tdyn_gal : kMpcPerKmsToGyr * r_gal * (1.0 / kC);
+    const double iso_rel = thr - kEpsIso, est_rel = thr - est;
+    const double ratio = iso_rel * rcp(est_rel);
+    const double dimless = iso_rel > kDimMax * est_rel ? kDimMax : ratio * ratio;
+    const double inv_tau = rcp(fmax(t_dyn, P[GAL_P_BAR_TIMESCALE_MINIMUM]) * dimless);
+    const double rg = unstable ? fmax(0.0, mg_d) * inv_tau : 0.0;
+    const double rs = unstable ? fmax(0.0, ms_d) * inv_tau : 0.0;
+    const double rj = unstable ? fmax(0.0, j_d) * inv_tau : 0.0;
+    const double rzg = unstable ? fmax(0.0, zg_d) * inv_tau : 0.0;
+    const double rzs = unstable ? fmax(0.0, y[GAL_V_DISK_ABUNDANCES_STELLAR]) * inv_tau : 0.0;
+    f[GAL_V_DISK_MASS_GAS] -= rg;
+    f[GAL_V_SPHEROID_MASS_GAS] += rg;
+    f[GAL_V_DISK_MASS_STELLAR] -= rs;
+    f[GAL_V_SPHEROID_MASS_STELLAR] += rs;
+    f[GAL_V_DISK_ANGULAR_MOMENTUM] -= (1.0 - P[GAL_P_BAR_J_RETAINED_DISK]) * rj;
+    f[GAL_V_SPHEROID_ANGULAR_MOMENTUM] += P[GAL_P_BAR_J_RETAINED_SPHEROID] * rj;
+    f[GAL_V_DISK_ABUNDANCES_GAS] -= rzg;
+    f[GAL_V_SPHEROID_ABUNDANCES_GAS] += rzg;
+    f[GAL_V_DISK_ABUNDANCES_STELLAR] -= rzs;
+    f[GAL_V_SPHEROID_ABUNDANCES_STELLAR] += rzs;
   }
   // ---- hot-halo structure and cooling radius ----------------------------
   // beta=2/3 profile, core coreRadiusOverVirialRadius*r_vir, truncated at
@@ -358,9 +424,17 @@ __device__ __forceinline__ void rhs(const double* __restrict__ P, const DerivedC
   const double r_core = P[GAL_P_CORE_RADIUS_OVER_VIRIAL] * H.r_vir;
   const double inv_r_core = H.inv_r_vir * D.inv_core;
   const double xo = r_out * inv_r_core;
-  const double inv_shape = rcp(xo - atan(xo));
+  // shape integrals S = x - atan x and N = (x^2 - ln(1 + x^2))/2, by their
+  // Taylor series below x = 0.01 where the differences cancel
+  const double uo = xo * xo;
+  const bool small_xo = xo < 0.01;
+  const double shape_m = small_xo ? xo * uo * (1.0 / 3.0 - uo * (1.0 / 5.0 - uo * (1.0 / 7.0 - uo * (1.0 / 9.0))))
+                                  : xo - atan(xo);
+  const double shape_r = small_xo ? uo * uo * (1.0 / 4.0 - uo * (1.0 / 6.0 - uo * (1.0 / 8.0 - uo * (1.0 / 10.0))))
+                                  : 0.5 * uo - 0.5 * log1p(uo);
+  const double inv_shape = rcp(shape_m);
   const double rho0 = m_hot * inv_shape * (1.0 / (4.0 * kPi)) * inv_r_core * inv_r_core * inv_r_core;
-  const double r_mean = r_core * (0.5 * xo * xo - 0.5 * log1p(xo * xo)) * inv_shape;
+  const double r_mean = r_core * shape_r * inv_shape;
   // composition of metallicity-only abundances, fully ionized H and He
   constexpr double kNorm = 1.0 / (kHydrogenPrimordial + kHeliumPrimordial);
   const double X = (1.0 - zf) * kHydrogenPrimordial * kNorm;
@@ -450,6 +524,29 @@ __device__ __forceinline__ void rhs(const double* __restrict__ P, const DerivedC
     f[GAL_V_HOT_ANGULAR_MOMENTUM] -= hs ? mdot_hot * (y[GAL_V_HOT_ANGULAR_MOMENTUM] * inv_m_hot) : 0.0;
     f[GAL_V_BH_SPIN] += acc_ok ? spin_up : 0.0;
   }
+  // ---- blackHolesWinds (black_holes/winds.F90:102-133) with ciotti2009
+  // (black_holes/winds/Ciotti2009.F90:113-193): the wind power enters the
+  // spheroid gas, which drives an outflow efficiencyEnergeticOutflow P / V^2
+  // into the hot halo (spheroid/standard/_class.F90:673-725).
+  {
+    const double eff = P[GAL_P_BH_WIND_EFFICIENCY] * (P[GAL_P_BH_WIND_SCALES_WITH_RADIATIVE] != 0.0 ? eff_thin : 1.0);
+    const bool on = (ops & GAL_OPERATOR_BLACK_HOLE_WINDS) && acc_ok && eff > 0.0;
+    const double x = D.wind_pressure_ratio * mg_s * rcp(r_gal * eff * mdot) - 0.5;
+    double coupled = x <= 0.0 ? 0.0 : (x >= 1.0 ? 1.0 : x * x * (3.0 - 2.0 * x));
+    coupled = (mg_s > 0.0 && r_gal > 0.0) ? coupled : 0.0;
+    constexpr double kC2 = (kSpeedLight / kKilo) * (kSpeedLight / kKilo);
+    const double power = eff * coupled * mdot * kC2;
+    const bool drive = on && power > 0.0 && mg_s > 0.0 && mt_s > 0.0 && v_gal > 0.0;
+    const double mo = drive ? P[GAL_P_SPHEROID_ENERGETIC_OUTFLOW] * power * H.inv_v * H.inv_v : 0.0;
+    const double jo = mo * inv_mt_s * j_s;
+    const double zo = mo * inv_mg_s * zg_s;
+    f[GAL_V_SPHEROID_MASS_GAS] -= mo;
+    f[GAL_V_SPHEROID_ANGULAR_MOMENTUM] -= jo;
+    f[GAL_V_SPHEROID_ABUNDANCES_GAS] -= zo;
+    f[GAL_V_HOT_OUTFLOWED_MASS] += mo;
+    f[GAL_V_HOT_OUTFLOWED_ANGULAR_MOMENTUM] += jo * D.inv_one_minus_j_loss;
+    f[GAL_V_HOT_OUTFLOWED_ABUNDANCES] += zo;
+  }
   // ---- CGMAccretion (circumgalactic_medium/accretion.F90:517) with
   //      accretionHalo simple (accretion/halo/simple.F90:281-378) ---------
   {
@@ -494,8 +591,10 @@ __device__ __forceinline__ void rhs(const double* __restrict__ P, const DerivedC
     const double heat = P[GAL_P_EFFICIENCY_RADIO_MODE] * p_jet * sq(H.inv_v);
     const bool heating = ok && heat > cool;
     const bool cooling = ok && cool > heat;
-    // net heating expels gas into the outflowed reservoir (hotHaloMassEjection)
-    const double out = heating ? fmin(heat - cool, P[GAL_P_RATE_MAXIMUM_EXPULSION] * m_hot * H.inv_t_dyn) : 0.0;
+    // excess heating expels gas from the node (excessHeatDrivesOutflow); a
+    // satellite's joins its stripped reservoir (outflow_stripping/standard.F90:120-131)
+    const bool expel = heating && P[GAL_P_EXCESS_HEAT_DRIVES_OUTFLOW] != 0.0;
+    const double out = expel ? fmin(heat - cool, P[GAL_P_RATE_MAXIMUM_EXPULSION] * m_hot * H.inv_t_dyn) : 0.0;
     // net cooling onto the disk; constantRotation j(r) = (J/M) r / <r>
     const double c = cooling ? fmax(cool - heat, 0.0) : 0.0;
     const double j_hot = y[GAL_V_HOT_ANGULAR_MOMENTUM] * inv_m_hot;
@@ -507,12 +606,54 @@ __device__ __forceinline__ void rhs(const double* __restrict__ P, const DerivedC
     f[GAL_V_HOT_MASS] -= heating ? out : c;
     f[GAL_V_HOT_ABUNDANCES] -= heating ? oz : cz;
     f[GAL_V_HOT_ANGULAR_MOMENTUM] -= heating ? oj : cj;
-    f[GAL_V_HOT_OUTFLOWED_MASS] += out;
-    f[GAL_V_HOT_OUTFLOWED_ABUNDANCES] += heating ? oz : 0.0;
-    f[GAL_V_HOT_OUTFLOWED_ANGULAR_MOMENTUM] += heating ? oj : 0.0;
+    f[GAL_V_HOT_STRIPPED_MASS] += np.satellite ? out : 0.0;
+    f[GAL_V_HOT_STRIPPED_ABUNDANCES] += np.satellite ? oz : 0.0;
     f[GAL_V_DISK_MASS_GAS] += c;
     f[GAL_V_DISK_ABUNDANCES_GAS] += cz;
     f[GAL_V_DISK_ANGULAR_MOMENTUM] += cj * (1.0 - P[GAL_P_FRACTION_LOSS_ANGULAR_MOMENTUM]);
+  }
+  // ---- CGMOuterRadiusRamPressureStripping
+  //      (circumgalactic_medium/outer_radius/ram_pressure_stripping.F90:111-242)
+  // Returning outflows push the outer radius out when it lies inside the
+  // virial radius; a central's follows the virial radius; a satellite's
+  // shrinks towards the font2008 stripping radius
+  // (hot_halo/ram_pressure_stripping/Font2008.F90:115-267) on the
+  // ram-pressure acceleration time (timescale/ram_pressure_acceleration.F90:79-134).
+  // The stripping radius is the closed form of the binding condition
+  // formFactor G M(<r) rho_hot(r) / r = F for the isothermal halo
+  // M(<r) = M_vir r / r_vir: formFactor V_vir^2 rho_hot(r) = F.
+  if (ops & GAL_OPERATOR_CGM_OUTER_RADIUS) {
+    const double rv = H.r_vir;
+    const double rho_ro = hot_ok ? rho_out : 0.0;
+    const double m_return = y[GAL_V_HOT_OUTFLOWED_MASS] * P[GAL_P_REINCORPORATION_MULTIPLIER] * H.inv_t_dyn;
+    const double rho_min = D.f_baryon * H.mass * (H.inv_r_vir * H.inv_r_vir * H.inv_r_vir) * (1.0 / (4.0 * kPi));
+    const double ret_a = m_return * (1.0 / (4.0 * kPi)) * rcp(r_out * r_out * fmax(rho_ro, rho_min));
+    const double ret_b = m_return * rcp(H.mass) * rv;
+    double ret = (r_out > 0.0 && rho_ro > 0.0) ? ret_a : (m_return > 0.0 ? ret_b : 0.0);
+    ret = r_out < rv ? ret : 0.0;
+    // stripping radius and timescale
+    const double force = np.ram_force;
+    const double bind = P[GAL_P_RAM_PRESSURE_FORM_FACTOR] * H.v_vir * H.v_vir;
+    const double x_v = rv * inv_r_core, r_small = 1.0e-6 * rv, x_s = r_small * inv_r_core;
+    const double rho_rv = (hot_ok && rv <= r_out) ? rho0 * rcp(1.0 + x_v * x_v) : 0.0;
+    const double rho_small = (hot_ok && r_small <= r_out) ? rho0 * rcp(1.0 + x_s * x_s) : 0.0;
+    const double r_star = fmin(r_core * sqrt(fmax(bind * rho0 * rcp(force) - 1.0, 0.0)), r_out);
+    const double r_rp = bind * rho_rv - force >= 0.0 ? rv : (bind * rho_small - force <= 0.0 ? 0.0 : r_star);
+    const double sigma = r_out * rho_ro;
+    const double t_floor = np.v_host > 0.0 ? r_out * 0.1 * rcp(np.v_host) : HUGE_VAL;
+    double tau = kMpcPerKmsToGyr * fmax(sqrt(2.0 * r_out * sigma * rcp(force)), t_floor);
+    tau = (r_out <= 0.0 || sigma <= 0.0) ? H.t_dyn : tau;
+    tau = force <= 0.0 ? kDoubleMax : tau;
+    const double dr = (r_rp < r_out && y[GAL_V_HOT_ANGULAR_MOMENTUM] > 0.0) ? (r_rp - r_out) / tau : 0.0;
+    const bool strip = np.satellite && dr != 0.0 && m_hot > 0.0 && r_out > 1.0e-3 * rv;
+    const double loss = strip ? 4.0 * kPi * rho_ro * r_out * r_out * dr : 0.0;
+    const double lm = loss * inv_m_hot;
+    f[GAL_V_HOT_OUTER_RADIUS] += ret + (np.satellite ? (strip ? dr : 0.0) : H.r_vir_rate);
+    f[GAL_V_HOT_MASS] += loss;
+    f[GAL_V_HOT_ANGULAR_MOMENTUM] += y[GAL_V_HOT_ANGULAR_MOMENTUM] * lm;
+    f[GAL_V_HOT_ABUNDANCES] += y[GAL_V_HOT_ABUNDANCES] * lm;
+    f[GAL_V_HOT_STRIPPED_MASS] -= loss;
+    f[GAL_V_HOT_STRIPPED_ABUNDANCES] -= y[GAL_V_HOT_ABUNDANCES] * lm;
   }
   // ---- coolingEnergyRadiated (cooling_energy_radiated.F90:200-260) -------
   {
@@ -616,10 +757,12 @@ __device__ __noinline__ void post_step_fix(const double* __restrict__ P, HaloCon
   D.rvir_coeff = hc.rvir_coeff;
   const Halo H = halo(P, D, np, t, y[GAL_V_SPIN_ANGULAR_MOMENTUM * stride]);
   const double rv = H.r_gal * H.v_vir;
-  component_post_step(y, stride, GAL_V_DISK_MASS_STELLAR, GAL_V_DISK_ABUNDANCES_STELLAR, GAL_V_DISK_MASS_GAS,
-                      GAL_V_DISK_ABUNDANCES_GAS, GAL_V_DISK_ANGULAR_MOMENTUM, rv);
-  component_post_step(y, stride, GAL_V_SPHEROID_MASS_STELLAR, GAL_V_SPHEROID_ABUNDANCES_STELLAR,
-                      GAL_V_SPHEROID_MASS_GAS, GAL_V_SPHEROID_ABUNDANCES_GAS, GAL_V_SPHEROID_ANGULAR_MOMENTUM, rv);
+  if (P[GAL_P_POST_STEP_ZERO_NEGATIVE_MASSES] != 0.0) {
+    component_post_step(y, stride, GAL_V_DISK_MASS_STELLAR, GAL_V_DISK_ABUNDANCES_STELLAR, GAL_V_DISK_MASS_GAS,
+                        GAL_V_DISK_ABUNDANCES_GAS, GAL_V_DISK_ANGULAR_MOMENTUM, rv);
+    component_post_step(y, stride, GAL_V_SPHEROID_MASS_STELLAR, GAL_V_SPHEROID_ABUNDANCES_STELLAR,
+                        GAL_V_SPHEROID_MASS_GAS, GAL_V_SPHEROID_ABUNDANCES_GAS, GAL_V_SPHEROID_ANGULAR_MOMENTUM, rv);
+  }
   if (y[GAL_V_HOT_MASS * stride] < 0.0) y[GAL_V_HOT_MASS * stride] = 0.0;
   if (y[GAL_V_HOT_OUTER_RADIUS * stride] < 0.0) y[GAL_V_HOT_OUTER_RADIUS * stride] = 0.0;
 }
@@ -627,11 +770,13 @@ __device__ __noinline__ void post_step_fix(const double* __restrict__ P, HaloCon
 __device__ __forceinline__ void post_step(const double* __restrict__ P, const DerivedConstants& D, const NodePar& np,
                                           int mode, double t, double* __restrict__ y, int stride) {
   if (mode == GAL_PHYSICS_TEST) return;
-  const bool neg = y[GAL_V_DISK_MASS_GAS * stride] < 0.0 || y[GAL_V_DISK_MASS_STELLAR * stride] < 0.0 ||
-                   y[GAL_V_DISK_ANGULAR_MOMENTUM * stride] < 0.0 || y[GAL_V_SPHEROID_MASS_GAS * stride] < 0.0 ||
-                   y[GAL_V_SPHEROID_MASS_STELLAR * stride] < 0.0 ||
-                   y[GAL_V_SPHEROID_ANGULAR_MOMENTUM * stride] < 0.0 || y[GAL_V_HOT_MASS * stride] < 0.0 ||
-                   y[GAL_V_HOT_OUTER_RADIUS * stride] < 0.0;
+  const bool galaxy = P[GAL_P_POST_STEP_ZERO_NEGATIVE_MASSES] != 0.0;
+  const bool neg = (galaxy && (y[GAL_V_DISK_MASS_GAS * stride] < 0.0 || y[GAL_V_DISK_MASS_STELLAR * stride] < 0.0 ||
+                               y[GAL_V_DISK_ANGULAR_MOMENTUM * stride] < 0.0 ||
+                               y[GAL_V_SPHEROID_MASS_GAS * stride] < 0.0 ||
+                               y[GAL_V_SPHEROID_MASS_STELLAR * stride] < 0.0 ||
+                               y[GAL_V_SPHEROID_ANGULAR_MOMENTUM * stride] < 0.0)) ||
+                   y[GAL_V_HOT_MASS * stride] < 0.0 || y[GAL_V_HOT_OUTER_RADIUS * stride] < 0.0;
   if (neg) post_step_fix(P, HaloConsts{D.om, D.ol, D.a3_norm, D.sinh_arg, D.rvir_coeff}, np, t, y, stride);
 }
 

@@ -459,7 +459,7 @@ __global__ void loss_sum_kernel(const float* __restrict__ loss_partial, int nb, 
 }
 
 // ---- node encoding ------------------------------------------------------------
-enum Role : int { R_MASS, R_METAL, R_ANGMOM, R_ENERGY, R_BH, R_UNIT, R_CONST, R_SPIN };
+enum Role : int { R_MASS, R_METAL, R_ANGMOM, R_ENERGY, R_BH, R_UNIT, R_CONST, R_SPIN, R_FIRST_ORDER };
 
 __device__ __forceinline__ int var_role(int v) {
   switch (v) {
@@ -478,6 +478,11 @@ __device__ __forceinline__ int var_role(int v) {
     case GAL_V_BH_MASS: return R_BH;
     case GAL_V_BH_SPIN: return R_UNIT;
     case GAL_V_SPIN_ANGULAR_MOMENTUM: return R_SPIN;
+    // not fed to the network: advanced by the first-order step of the rate at
+    // the interval's start (outer radius, bound mass and stripped gas)
+    case GAL_V_HOT_OUTER_RADIUS: case GAL_V_SATELLITE_BOUND_MASS: case GAL_V_HOT_STRIPPED_MASS:
+    case GAL_V_HOT_STRIPPED_ABUNDANCES:
+      return R_FIRST_ORDER;
     default: return R_CONST;
   }
 }
@@ -540,7 +545,7 @@ __device__ __forceinline__ void euler_baseline(const double* __restrict__ P, con
     double d = 0.0;
     if (role == R_ENERGY) {
       d = asinh(rate[v] * dt / f.s_energy);
-    } else if (role != R_CONST && role != R_SPIN) {
+    } else if (role != R_CONST && role != R_SPIN && role != R_FIRST_ORDER) {
       const double s = var_scale(f, role);
       if (rate[v] >= 0.0)
         d = asinh((y0[v] + rate[v] * dt) / s) - asinh(y0[v] / s);          // linear growth
@@ -606,7 +611,7 @@ __global__ __launch_bounds__(256) void encode_kernel(int n, int ld, const double
       double d = 0.0;
       if (role == R_ENERGY) {
         d = asinh((b - a) / f.s_energy) - du[v];
-      } else if (role != R_CONST && role != R_SPIN) {
+      } else if (role != R_CONST && role != R_SPIN && role != R_FIRST_ORDER) {
         const double s = var_scale(f, role);
         d = asinh(b / s) - asinh(a / s) - du[v];
       }
@@ -666,6 +671,8 @@ __global__ __launch_bounds__(256) void apply_kernel(GateArgs a) {
         const int role = var_role(v);
         if (role == R_CONST) {
           y1[v] = y0[v];
+        } else if (role == R_FIRST_ORDER) {
+          y1[v] = y0[v] + rate[v] * dt;
         } else if (role == R_SPIN) {
           y1[v] = y0[v] + (np.satellite ? 0.0 : np.spin_rate) * dt;
         } else {
@@ -709,13 +716,13 @@ __global__ __launch_bounds__(256) void apply_kernel(GateArgs a) {
       const double accreted = np.satellite ? 0.0 : a.D.f_baryon * np.m_rate * dt;
       auto baryons = [](const double* s) {
         return s[GAL_V_HOT_MASS] + s[GAL_V_HOT_OUTFLOWED_MASS] + s[GAL_V_HOT_UNACCRETED_MASS] +
-               s[GAL_V_DISK_MASS_GAS] + s[GAL_V_DISK_MASS_STELLAR] + s[GAL_V_SPHEROID_MASS_GAS] +
-               s[GAL_V_SPHEROID_MASS_STELLAR] + s[GAL_V_BH_MASS];
+               s[GAL_V_HOT_STRIPPED_MASS] + s[GAL_V_DISK_MASS_GAS] + s[GAL_V_DISK_MASS_STELLAR] +
+               s[GAL_V_SPHEROID_MASS_GAS] + s[GAL_V_SPHEROID_MASS_STELLAR] + s[GAL_V_BH_MASS];
       };
       auto metals = [](const double* s) {
         return s[GAL_V_HOT_ABUNDANCES] + s[GAL_V_HOT_OUTFLOWED_ABUNDANCES] + s[GAL_V_HOT_UNACCRETED_ABUNDANCES] +
-               s[GAL_V_DISK_ABUNDANCES_GAS] + s[GAL_V_DISK_ABUNDANCES_STELLAR] + s[GAL_V_SPHEROID_ABUNDANCES_GAS] +
-               s[GAL_V_SPHEROID_ABUNDANCES_STELLAR];
+               s[GAL_V_HOT_STRIPPED_ABUNDANCES] + s[GAL_V_DISK_ABUNDANCES_GAS] + s[GAL_V_DISK_ABUNDANCES_STELLAR] +
+               s[GAL_V_SPHEROID_ABUNDANCES_GAS] + s[GAL_V_SPHEROID_ABUNDANCES_STELLAR];
       };
       const double mb0 = baryons(y0), mb1 = baryons(y1);
       const double dbh = fmax(y1[GAL_V_BH_MASS] - y0[GAL_V_BH_MASS], 0.0);

@@ -428,7 +428,7 @@ struct Walk {
       : f(ff), c(cc), w(ww), ops(o), lo(l), hi(h) {
     const int64_t n = hi - lo;
     parent.resize(n); first_child.resize(n); sibling.resize(n); first_sat.assign(n, -1);
-    is_sat.assign(n, 0); alive.assign(n, 1); time.resize(n); merge_time.assign(n, kInf); par.resize(5 * n);
+    is_sat.assign(n, 0); alive.assign(n, 1); time.resize(n); merge_time.assign(n, kInf); par.assign((size_t)GAL_NPARAM_NODE * n, 0.0);
     for (int64_t i = 0; i < n; ++i) {
       auto loc = [&](int32_t g) { return g < 0 ? -1 : (int32_t)(g - lo); };
       parent[i] = loc(f.parent[lo + i]);
@@ -454,7 +454,7 @@ struct Walk {
         m_rate = (mp - m) / (tp - t);
         j_rate = (jp - j) / (tp - t);
       }
-      double* q = &par[5 * i];
+      double* q = &par[(size_t)GAL_NPARAM_NODE * i];
       q[GAL_NP_TIME_REFERENCE] = t;
       q[GAL_NP_MASS_REFERENCE] = m;
       q[GAL_NP_MASS_RATE] = m_rate;
@@ -466,7 +466,7 @@ struct Walk {
   int32_t glob(int32_t i) const { return i < 0 ? -1 : (int32_t)(lo + i); }
   bool primary(int32_t n) const { return parent[n] >= 0 && first_child[parent[n]] == n; }
   double halo_mass(int32_t n, double t) const {
-    const double* q = &par[5 * n];
+    const double* q = &par[(size_t)GAL_NPARAM_NODE * n];
     return q[GAL_NP_MASS_REFERENCE] + q[GAL_NP_MASS_RATE] * (t - q[GAL_NP_TIME_REFERENCE]);
   }
   int32_t descend(int32_t n) const {
@@ -514,7 +514,7 @@ struct Walk {
     op.wave = 0;
     op.t0 = t0;
     op.t1 = t1;
-    if (np) for (int k = 0; k < 5; ++k) op.node_par[k] = np[k];
+    if (np) for (int k = 0; k < GAL_NPARAM_NODE; ++k) op.node_par[k] = np[k];
     ops.push_back(op);
   }
   // Jiang, Jing, Faltenbacher, Lin & Li (2008) merging time
@@ -617,7 +617,7 @@ struct Walk {
           const double t_end = evolve_to(node, merge_task);
           if (t_end > time[node]) {
             progressed = true;
-            emit(GAL_OP_EVOLVE, node, -1, time[node], t_end, &par[5 * node]);
+            emit(GAL_OP_EVOLVE, node, -1, time[node], t_end, &par[(size_t)GAL_NPARAM_NODE * node]);
             time[node] = t_end;
           }
           if (merge_task) {
@@ -642,7 +642,7 @@ struct Walk {
                 move_satellites(node, p);
                 is_sat[node] = 1;
                 merge_time[node] = time[node] + merging_time(first_child[p], node, time[node]);
-                double* q = &par[5 * node];
+                double* q = &par[(size_t)GAL_NPARAM_NODE * node];
                 q[GAL_NP_MASS_REFERENCE] = halo_mass(node, time[node]);
                 q[GAL_NP_TIME_REFERENCE] = time[node];
                 q[GAL_NP_MASS_RATE] = 0.0;

@@ -105,12 +105,30 @@ class NodeEvolverBatched:
                    "gal_ode_batch_set_launch")
 
     # -- evolution ---------------------------------------------------------
+    @staticmethod
+    def _check_shapes(n, y, node_par, *per_node, scale=None):
+        """The kernels address y[v*ld + i] and node_par[k*ld + i] with ld =
+        y.shape[1]: reject arrays that would put those addresses out of bounds."""
+        ld = int(y.shape[1])
+        if tuple(y.shape) != (gp.NVAR, ld) or n > ld:
+            raise ValueError(f"y must have shape ({gp.NVAR}, ld >= n)")
+        if tuple(node_par.shape) != (gp.NPARAM_NODE, ld):
+            raise ValueError(f"node_par must have shape ({gp.NPARAM_NODE}, {ld})")
+        if scale is not None and tuple(scale.shape) != (gp.NVAR, ld):
+            raise ValueError(f"scale must have shape ({gp.NVAR}, {ld})")
+        for a in per_node:
+            if a is not None and int(a.shape[0]) < n:
+                raise ValueError(f"per-node arrays need at least {n} entries")
+
     def evolve_device(self, y, node_par, t_start, t_end, h_step, scale=None, status=None, counters=None,
                       stream=None):
         """Evolve in place on device tensors (float64 SoA, y: (30, ld))."""
         import torch
         n = int(t_start.shape[0])
         ld = int(y.shape[1])
+        self._check_shapes(n, y, node_par, t_end, h_step, status, scale=scale)
+        if counters is not None and int(counters.numel()) < 3 * n:
+            raise ValueError(f"counters need {3 * n} entries")
         if status is None:
             status = torch.empty(n, dtype=torch.int32, device=y.device)
         if scale is None:
@@ -135,6 +153,7 @@ class NodeEvolverBatched:
         npar = np.ascontiguousarray(node_par, dtype=np.float64)
         ts = np.ascontiguousarray(t_start, dtype=np.float64)
         te = np.ascontiguousarray(t_end, dtype=np.float64)
+        self._check_shapes(n, y, npar, ts, te, h, scale=sc)
         _lib.check(self.L.gal_ode_batch_evolve_h(self.handle, n, y.ctypes.data_as(dp), npar.ctypes.data_as(dp),
                                                  ts.ctypes.data_as(dp), te.ctypes.data_as(dp),
                                                  h.ctypes.data_as(dp), sc.ctypes.data_as(dp),
@@ -146,6 +165,7 @@ class NodeEvolverBatched:
         import torch
         if dydt is None:
             dydt = torch.empty_like(y)
+        self._check_shapes(int(t.shape[0]), y, node_par, scale=dydt)
         s = ctypes.c_void_p(0 if stream is None else stream.cuda_stream)
         _lib.check(self.L.gal_ode_rhs_d(self.handle, int(t.shape[0]), int(y.shape[1]), _ptr(t), _ptr(y),
                                         _ptr(node_par), _ptr(dydt), s), "gal_ode_rhs_d")
@@ -155,6 +175,7 @@ class NodeEvolverBatched:
         """Time the launch shapes on copies of the inputs; keeps the fastest."""
         import torch
         n = int(t_start.shape[0])
+        self._check_shapes(n, y, node_par, t_end, h_step)
         bn, bb = ctypes.c_int(), ctypes.c_int()
         scale = torch.empty_like(y)
         status = torch.empty(n, dtype=torch.int32, device=y.device)

@@ -3,7 +3,7 @@
 !! Galacticus_AMD_ODE binding and writes the result.
 !!
 !! Input (stream, native endianness): int32 n, nT, nZ, flags(reuse, nonneg);
-!! float64 tolAbs, tolRel, lnTMin, dLnT, lnZMin, dLnZ; params(35);
+!! float64 tolAbs, tolRel, lnTMin, dLnT, lnZMin, dLnZ; params(48);
 !! table(nT*nZ); y(n*30); nodePar(n*5); tStart(n); tEnd(n); timeStep(n).
 !! Output: y(n*30), timeStep(n), scale(n*30), int32 status(n), int32 return code.
 program evolve_batch_file

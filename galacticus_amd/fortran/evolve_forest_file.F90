@@ -5,7 +5,7 @@
 !! Galacticus_AMD_Trees.
 !!
 !! Input (stream, native endianness): int32 countTrees, nT, nZ; int64 seed;
-!! float64 tolAbs, tolRel, lnTMin, dLnT, lnZMin, dLnZ; params(35);
+!! float64 tolAbs, tolRel, lnTMin, dLnT, lnZMin, dLnZ; params(48);
 !! table(nT*nZ); massRoot(countTrees); timeRoot; cosmology(7); pch(8);
 !! walk(11); metalFraction, blackHoleSeed, majorRatio.
 !! Output: int64 countNodes; int64 treeOffset(countTrees+1);

@@ -14,8 +14,8 @@ module Galacticus_AMD_ODE
   public :: nodeEvolverBatched, galVariableCount, galParameterCount, galNodeParameterCount
 
   integer(c_int), parameter :: galVariableCount     =30
-  integer(c_int), parameter :: galParameterCount    =35
-  integer(c_int), parameter :: galNodeParameterCount= 5
+  integer(c_int), parameter :: galParameterCount    =48
+  integer(c_int), parameter :: galNodeParameterCount= 7
   integer(c_int), parameter :: flagEnforceNonNegativity=1, flagReuseStepSize=2, flagCostOrder=8
 
   interface
@@ -103,7 +103,7 @@ contains
   end function batchedSetCoolingTable
 
   integer function batchedEvolve(self,y,nodeParameters,timeStart,timeEnd,timeStep,scale,statusNode) result(status)
-    !! Evolve all nodes; y(n,30), nodeParameters(n,5) (column-major = the engine's structure of arrays).
+    !! Evolve all nodes; y(n,30), nodeParameters(n,7) (column-major = the engine's structure of arrays).
     class           (nodeEvolverBatched), intent(inout)                 :: self
     double precision                    , intent(inout), dimension(:,:) :: y
     double precision                    , intent(in   ), dimension(:,:) :: nodeParameters

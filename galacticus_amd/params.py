@@ -24,7 +24,18 @@ parameter-XML element names; defaults are those of ``parameters/quickTest.xml``
 * black holes: Bondi-Hoyle enhancements 5 (spheroid) and 6 (hot halo), spheroid
   gas temperature 100 K (quickTest.xml:198-203);
 * accretion: reionization z=10.5, V=35 km/s (quickTest.xml:101-104);
-* structure: fixed solver factor sqrt(1/2) (galactic_structure/radius_solver/fixed.F90:97-106).
+* structure: fixed solver factor sqrt(1/2) (galactic_structure/radius_solver/fixed.F90:97-106);
+* the optional operators of the nodeOperator list (quickTest.xml:233-288 lists
+  all four): satelliteMassLoss, barInstability with efstathiou1982 thresholds
+  1.1 (stellar) and 0.7 (gaseous) (quickTest.xml:259-264) and the class
+  defaults timescaleMinimum 1e-9 Gyr and retained angular momentum fractions
+  1 (galactic_dynamics/bar_instability/Efstathiou1982.F90:89-128),
+  blackHolesWinds with ciotti2009 efficiencyWind 0.0024 scaled by the
+  radiative efficiency (quickTest.xml:174-177) and the spheroid's
+  efficiencyEnergeticOutflow 0.01 (spheroid/standard/_class.F90:225-232),
+  CGMOuterRadiusRamPressureStripping with the font2008 formFactor 2
+  (hot_halo/ram_pressure_stripping/Font2008.F90:90-97);
+* CGMCoolingHeating excessHeatDrivesOutflow true (quickTest.xml:285).
 """
 from __future__ import annotations
 
@@ -34,7 +45,7 @@ import xml.etree.ElementTree as ET
 import numpy as np
 
 NVAR = 30
-NPARAM_NODE = 5
+NPARAM_NODE = 7
 
 # Indices mirror enum gal_param in include/gal_ode.h.
 PARAM_NAMES = [
@@ -73,6 +84,19 @@ PARAM_NAMES = [
     "toleranceRelativeMetallicity",
     "starFormationTimescale.fixed.timescale",
     "stellarFeedbackOutflows.fixed.fraction",
+    "nodeOperators",
+    "CGMCoolingHeating.excessHeatDrivesOutflow",
+    "barInstability.stabilityThresholdStellar",
+    "barInstability.stabilityThresholdGaseous",
+    "barInstability.timescaleMinimum",
+    "barInstability.fractionAngularMomentumRetainedDisk",
+    "barInstability.fractionAngularMomentumRetainedSpheroid",
+    "blackHoleWind.efficiencyWind",
+    "blackHoleWind.efficiencyWindScalesWithEfficiencyRadiative",
+    "componentSpheroid.efficiencyEnergeticOutflow",
+    "hotHaloRamPressureStripping.formFactor",
+    "stellarFeedbackOutflows.fixed.fractionSpheroid",
+    "postStepZeroNegativeMasses",
 ]
 P = {name: i for i, name in enumerate(PARAM_NAMES)}
 NPARAM = len(PARAM_NAMES)
@@ -117,6 +141,28 @@ DEFAULTS = {
     "toleranceRelativeMetallicity": 1.0e-4,
     "starFormationTimescale.fixed.timescale": 1.0,
     "stellarFeedbackOutflows.fixed.fraction": 0.0,
+    "nodeOperators": 0xF,
+    "CGMCoolingHeating.excessHeatDrivesOutflow": 1.0,
+    "barInstability.stabilityThresholdStellar": 1.1,
+    "barInstability.stabilityThresholdGaseous": 0.7,
+    "barInstability.timescaleMinimum": 1.0e-9,
+    "barInstability.fractionAngularMomentumRetainedDisk": 1.0,
+    "barInstability.fractionAngularMomentumRetainedSpheroid": 1.0,
+    "blackHoleWind.efficiencyWind": 0.0024,
+    "blackHoleWind.efficiencyWindScalesWithEfficiencyRadiative": 1.0,
+    "componentSpheroid.efficiencyEnergeticOutflow": 1.0e-2,
+    "hotHaloRamPressureStripping.formFactor": 2.0,
+    "stellarFeedbackOutflows.fixed.fractionSpheroid": 0.0,
+    "postStepZeroNegativeMasses": 1.0,
+}
+
+# Optional node operators (GAL_OPERATOR_* bits of include/gal_ode.h), by the
+# nodeOperator value that lists them in a parameter file.
+OPERATOR_BITS = {
+    "satelliteMassLoss": 0x1,
+    "barInstability": 0x2,
+    "blackHolesWinds": 0x4,
+    "CGMOuterRadiusRamPressureStripping": 0x8,
 }
 
 # Node-evolver keys and defaults (merger_trees/node_evolver/standard.F90:165-256).
@@ -175,7 +221,28 @@ _XML_MAP = {
     "accretionHalo/velocitySuppressionReionization": "velocitySuppressionReionization",
     "galacticStructureSolver/factor": "galacticStructureSolver.factor",
     "componentDisk/toleranceRelativeMetallicity": "toleranceRelativeMetallicity",
+    "galacticDynamicsBarInstability/stabilityThresholdStellar": "barInstability.stabilityThresholdStellar",
+    "galacticDynamicsBarInstability/stabilityThresholdGaseous": "barInstability.stabilityThresholdGaseous",
+    "galacticDynamicsBarInstability/timescaleMinimum": "barInstability.timescaleMinimum",
+    "galacticDynamicsBarInstability/fractionAngularMomentumRetainedDisk":
+        "barInstability.fractionAngularMomentumRetainedDisk",
+    "galacticDynamicsBarInstability/fractionAngularMomentumRetainedSpheroid":
+        "barInstability.fractionAngularMomentumRetainedSpheroid",
+    "blackHoleWind/efficiencyWind": "blackHoleWind.efficiencyWind",
+    "componentSpheroid/efficiencyEnergeticOutflow": "componentSpheroid.efficiencyEnergeticOutflow",
+    "hotHaloRamPressureStripping/formFactor": "hotHaloRamPressureStripping.formFactor",
 }
+
+_BOOLEAN_XML = {
+    "blackHoleWind/efficiencyWindScalesWithEfficiencyRadiative":
+        "blackHoleWind.efficiencyWindScalesWithEfficiencyRadiative",
+    "componentDisk/postStepZeroNegativeMasses": "postStepZeroNegativeMasses",
+    "componentSpheroid/postStepZeroNegativeMasses": "postStepZeroNegativeMasses",
+}
+
+
+def _truth(value: str) -> bool:
+    return value.strip().lower() in ("true", ".true.", "1", "yes")
 
 
 def _walk(elem, path, out):
@@ -209,15 +276,22 @@ def parse_parameter_xml(path_or_text: str):
         for suffix, key in _XML_MAP.items():
             if path.endswith(suffix):
                 overrides[key] = float(value)
+        for suffix, key in _BOOLEAN_XML.items():
+            if path.endswith(suffix):
+                overrides[key] = 1.0 if _truth(value) else 0.0
         if path.startswith("mergerTreeNodeEvolver/"):
             k = path.split("/", 1)[1]
             if k in ("odeToleranceAbsolute", "odeToleranceRelative"):
                 evolver[k] = float(value)
             elif k in ("reuseODEStepSize", "enforceNonNegativity", "profileOdeEvolver"):
-                evolver[k] = value.strip().lower() in ("true", ".true.", "1", "yes")
+                evolver[k] = _truth(value)
             elif k == "odeAlgorithm":
                 evolver[k] = value
-    # Feedback and star-formation blocks nested under nodeOperator entries.
+    # Feedback and star-formation blocks nested under nodeOperator entries;
+    # the optional operators are on exactly when listed.
+    listed = {op.attrib.get("value") for op in root.iter("nodeOperator")}
+    if "multi" in listed:
+        overrides["nodeOperators"] = float(sum(bit for name, bit in OPERATOR_BITS.items() if name in listed))
     for op in root.iter("nodeOperator"):
         kind = op.attrib.get("value")
         if kind in ("stellarFeedbackDisks", "stellarFeedbackSpheroids"):
@@ -231,11 +305,15 @@ def parse_parameter_xml(path_or_text: str):
                     elif c.tag == "timescaleOutflowFractionalMinimum":
                         overrides[tag + ".timescaleOutflowFractionalMinimum"] = float(c.attrib["value"])
                     elif c.tag == "fraction":
-                        overrides["stellarFeedbackOutflows.fixed.fraction"] = float(c.attrib["value"])
+                        key = ("stellarFeedbackOutflows.fixed.fraction" if tag == "stellarFeedbackDisks"
+                               else "stellarFeedbackOutflows.fixed.fractionSpheroid")
+                        overrides[key] = float(c.attrib["value"])
         if kind == "CGMCoolingHeating":
             for c in op:
                 if c.tag == "rateMaximumExpulsion":
                     overrides["CGMCoolingHeating.rateMaximumExpulsion"] = float(c.attrib["value"])
+                elif c.tag == "excessHeatDrivesOutflow":
+                    overrides["CGMCoolingHeating.excessHeatDrivesOutflow"] = 1.0 if _truth(c.attrib["value"]) else 0.0
     if evolver["odeAlgorithm"] != "rungeKuttaCashKarp":
         raise ValueError("only odeAlgorithm=rungeKuttaCashKarp is implemented by the batched evolver")
     return physics_params(**overrides), evolver

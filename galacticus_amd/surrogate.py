@@ -177,7 +177,7 @@ class PairSet:
     """(state, interval -> state) pairs, device tensors, SoA like the evolver."""
     y0: "object"        # (30, n) float64
     y1: "object"
-    node_par: "object"  # (5, n)
+    node_par: "object"  # (GAL_NPARAM_NODE, n)
     t0: "object"
     t1: "object"
 

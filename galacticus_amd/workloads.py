@@ -5,7 +5,8 @@ systems used for the throughput measurement: halo mass log-uniform in
 [1e9, 1e13] Msun, initial hot-gas fraction uniform in [0.05, 0.17], hot-gas
 metallicity log-uniform in [1e-4, 0.02], interval length uniform in
 [0.1, 2] Gyr, cosmology Omega_m=0.3, Omega_L=0.7, h=0.7.  Every other initial
-component value is drawn from the same generator so a seed fixes the batch.
+component value is drawn from the same generator so a seed fixes the batch;
+disks start bar-stable.
 """
 from __future__ import annotations
 
@@ -31,7 +32,8 @@ V = dict(
     SPIN_ANGULAR_MOMENTUM=28, SATELLITE_BOUND_MASS=29,
 )
 VAR_NAMES = [k for k, _ in sorted(V.items(), key=lambda kv: kv[1])]
-NP = dict(TIME_REFERENCE=0, MASS_REFERENCE=1, MASS_RATE=2, SPIN_RATE=3, IS_SATELLITE=4)
+NP = dict(TIME_REFERENCE=0, MASS_REFERENCE=1, MASS_RATE=2, SPIN_RATE=3, IS_SATELLITE=4, RAM_PRESSURE_FORCE=5,
+          HOST_VELOCITY_VIRIAL=6)
 
 # Masses and metal masses (the quantities compared per component).
 MASS_VARS = [V[k] for k in ("HOT_MASS", "HOT_OUTFLOWED_MASS", "HOT_UNACCRETED_MASS", "HOT_STRIPPED_MASS",
@@ -79,7 +81,7 @@ def virial(mass, t, om=0.3, ol=0.7, h0=70.0):
 @dataclasses.dataclass
 class NodeBatch:
     y: np.ndarray          # (30, n)
-    node_par: np.ndarray   # (5, n)
+    node_par: np.ndarray   # (7, n): GAL_NPARAM_NODE rows
     t_start: np.ndarray
     t_end: np.ndarray
     h_step: np.ndarray
@@ -116,6 +118,20 @@ def isolated_halos(n: int, seed: int = 0, params=None) -> NodeBatch:
     r_disk = p[gp.P["galacticStructureSolver.factor"]] * r_vir * spin
     y = np.zeros((NV, n))
     mb = fb * mass
+    # Disks start below the bar-instability threshold (efstathiou1982: eps =
+    # 1.18 V / sqrt(G M_disk / r_disk) at or above the threshold interpolated
+    # in gas fraction) as a disk grown by cooling would: a random disk drawn
+    # above it is driven to the threshold, and the spheroid that it feeds
+    # (star formation on 1e-3 Gyr and outflows (100 km/s / V)^3.5 times faster
+    # in a dwarf) becomes a 1e-5 Gyr stiff reservoir.
+    m_disk = (f_dg + f_ds) * mb
+    f_gas = f_dg / (f_dg + f_ds)
+    thr = (p[gp.P["barInstability.stabilityThresholdStellar"]] * (1.0 - f_gas) +
+           p[gp.P["barInstability.stabilityThresholdGaseous"]] * f_gas)
+    m_stable = 0.9 * r_disk * v_vir ** 2 * (1.1800237580 / thr) ** 2 / G_INTERNAL
+    shrink = np.minimum(1.0, m_stable / m_disk)
+    f_dg = f_dg * shrink
+    f_ds = f_ds * shrink
     y[V["HOT_MASS"]] = f_hot * mass
     y[V["HOT_ABUNDANCES"]] = metal * y[V["HOT_MASS"]]
     y[V["HOT_ANGULAR_MOMENTUM"]] = y[V["HOT_MASS"]] * j_spin / mass
@@ -149,6 +165,38 @@ def isolated_halos(n: int, seed: int = 0, params=None) -> NodeBatch:
     return NodeBatch(y, npar, t0, t0 + dt, -np.ones(n))
 
 
+def satellite_halos(n: int, seed: int = 0, params=None) -> NodeBatch:
+    """The isolated-halo batch turned into satellites: each halo stopped
+    growing (became a satellite) up to 2 Gyr before its interval starts, its
+    host has a virial velocity 1.5-6 times its own, and the host's hot gas
+    presses on it with a ram pressure log-uniform in [1e-3, 0.3] times the
+    satellite's own binding scale formFactor V_vir^2 rho_0 of its hot-gas
+    profile, so that some halos keep their gas and some are stripped to a
+    radius inside their outer radius."""
+    p = gp.physics_params() if params is None else params
+    b = isolated_halos(n, seed, p)
+    rng = np.random.Generator(np.random.PCG64(seed + 7919))
+    om, ol, h0 = p[gp.P["OmegaMatter"]], p[gp.P["OmegaDarkEnergy"]], p[gp.P["HubbleConstant"]]
+    t_iso = b.t_start - rng.uniform(0.0, 2.0, n)
+    t_iso = np.maximum(t_iso, 0.5 * b.t_start)
+    mass = b.node_par[NP["MASS_REFERENCE"]]
+    r_vir, v_vir = virial(mass, t_iso, om, ol, h0)
+    y = b.y.copy()
+    y[V["HOT_OUTER_RADIUS"]] = r_vir * rng.uniform(0.3, 1.0, n)
+    rc = p[gp.P["coreRadiusOverVirialRadius"]] * r_vir
+    xo = y[V["HOT_OUTER_RADIUS"]] / rc
+    rho0 = y[V["HOT_MASS"]] / (4.0 * math.pi * rc ** 3 * (xo - np.arctan(xo)))
+    bind = p[gp.P["hotHaloRamPressureStripping.formFactor"]] * v_vir ** 2 * rho0
+    npar = b.node_par.copy()
+    npar[NP["TIME_REFERENCE"]] = t_iso
+    npar[NP["MASS_RATE"]] = 0.0
+    npar[NP["SPIN_RATE"]] = 0.0
+    npar[NP["IS_SATELLITE"]] = 1.0
+    npar[NP["RAM_PRESSURE_FORCE"]] = bind * 10.0 ** rng.uniform(-3.0, math.log10(0.3), n)
+    npar[NP["HOST_VELOCITY_VIRIAL"]] = v_vir * rng.uniform(1.5, 6.0, n)
+    return NodeBatch(y, npar, b.t_start, b.t_end, b.h_step)
+
+
 def closed_box(timescale=0.25, outflow=0.0, recycled=0.4, yield_=0.025):
     """The reference's closed/leaky-box single-node known answers
     (testSuite/test-reproducibility.py:47-68, testSuite/parameters/reproducibility/{closed,leaky}Box*.xml):
@@ -160,6 +208,34 @@ def closed_box(timescale=0.25, outflow=0.0, recycled=0.4, yield_=0.025):
     y = np.zeros((NV, 1))
     y[V["DISK_MASS_GAS"], 0] = 1.0e11
     y[V["DISK_FRACTION_MASS_RETAINED"], 0] = 1.0
+    y[V["SPIN_ANGULAR_MOMENTUM"], 0] = 1.635e12
+    y[V["SATELLITE_BOUND_MASS"], 0] = 1.0e12
+    npar = np.zeros((gp.NPARAM_NODE, 1))
+    npar[NP["TIME_REFERENCE"], 0] = 12.47
+    npar[NP["MASS_REFERENCE"], 0] = 1.0e12
+    return p, NodeBatch(y, npar, np.array([12.47]), np.array([13.47]), -np.ones(1))
+
+
+def rapid_depletion():
+    """The reference's non-negativity model (testSuite/parameters/enforceNonNegativity/
+    rapidDepletion.xml with tree.xml): a 1e12 Msun halo at t=12.47 Gyr whose disk
+    (1e11 Msun of stars, 1e10 of gas) and spheroid (1e9 Msun of gas) form stars on
+    a fixed 0.25 Gyr timescale with fixed outflow fractions 1 (disk) and 10
+    (spheroid), no hot halo, and no post-step zeroing of negative masses,
+    evolved to 13.47 Gyr."""
+    p = gp.physics_params(physicsMode=gp.PHYSICS_BOX, OmegaMatter=0.3, OmegaDarkEnergy=0.7, OmegaBaryon=0.05,
+                          HubbleConstant=70.0, recycledFraction=0.4, metalYield=0.025,
+                          postStepZeroNegativeMasses=0.0,
+                          **{"starFormationTimescale.fixed.timescale": 0.25,
+                             "stellarFeedbackOutflows.fixed.fraction": 1.0,
+                             "stellarFeedbackOutflows.fixed.fractionSpheroid": 10.0})
+    y = np.zeros((NV, 1))
+    y[V["DISK_MASS_STELLAR"], 0] = 1.0e11
+    y[V["DISK_MASS_GAS"], 0] = 1.0e10
+    y[V["DISK_ANGULAR_MOMENTUM"], 0] = 7.7e10
+    y[V["DISK_FRACTION_MASS_RETAINED"], 0] = 1.0
+    y[V["SPHEROID_MASS_GAS"], 0] = 1.0e9
+    y[V["SPHEROID_ANGULAR_MOMENTUM"], 0] = 7.7e8
     y[V["SPIN_ANGULAR_MOMENTUM"], 0] = 1.635e12
     y[V["SATELLITE_BOUND_MASS"], 0] = 1.0e12
     npar = np.zeros((gp.NPARAM_NODE, 1))

@@ -21,7 +21,7 @@ _LIB_PATH = os.path.join(_HERE, "lib", "liboracle.so")
 _lib = None
 
 NVAR = 30
-NPARAM_NODE = 5
+NPARAM_NODE = 7
 
 
 class OrcCooling(ctypes.Structure):

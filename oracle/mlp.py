@@ -213,14 +213,17 @@ def adam(p, m, v, g, step, lr0, lr_min, beta1, beta2, eps, total_steps):
 
 
 # ---- node encoding (restates the contract of include/gal_surrogate.h) -------
-ROLE_MASS, ROLE_METAL, ROLE_ANGMOM, ROLE_ENERGY, ROLE_BH, ROLE_UNIT, ROLE_CONST, ROLE_SPIN = range(8)
+ROLE_MASS, ROLE_METAL, ROLE_ANGMOM, ROLE_ENERGY, ROLE_BH, ROLE_UNIT, ROLE_CONST, ROLE_SPIN, ROLE_FIRST_ORDER = range(9)
+# ROLE_FIRST_ORDER: not fed to the network, advanced by the first-order step
+# of the rate at the interval's start (outer radius, stripped gas, bound mass)
 VAR_ROLES = [
     ROLE_MASS, ROLE_METAL, ROLE_ANGMOM, ROLE_MASS, ROLE_ANGMOM, ROLE_METAL, ROLE_MASS, ROLE_METAL,  # 0-7
-    ROLE_CONST, ROLE_CONST, ROLE_CONST, ROLE_ENERGY,                                              # 8-11
+    ROLE_FIRST_ORDER, ROLE_FIRST_ORDER, ROLE_FIRST_ORDER, ROLE_ENERGY,                            # 8-11
     ROLE_MASS, ROLE_MASS, ROLE_CONST, ROLE_METAL, ROLE_MASS, ROLE_METAL, ROLE_ANGMOM,             # 12-18
     ROLE_MASS, ROLE_MASS, ROLE_METAL, ROLE_MASS, ROLE_METAL, ROLE_ANGMOM,                         # 19-24
-    ROLE_BH, ROLE_UNIT, ROLE_CONST, ROLE_SPIN, ROLE_CONST,                                        # 25-29
+    ROLE_BH, ROLE_UNIT, ROLE_CONST, ROLE_SPIN, ROLE_FIRST_ORDER,                                  # 25-29
 ]
+_UNTRAINED = (ROLE_CONST, ROLE_SPIN, ROLE_FIRST_ORDER)
 INPUT_VARS = [0, 1, 2, 3, 4, 5, 6, 7, 12, 13, 15, 16, 17, 18, 19, 20, 21, 22, 23, 24, 25, 26]
 # every state variable but the angular momenta that can change sign, the
 # radiated energy, the black-hole radial position and the spin angular momentum
@@ -242,7 +245,7 @@ def _baseline(y0, rate, dt, sc):
         role = VAR_ROLES[v]
         if role == ROLE_ENERGY:
             du[v] = math.asinh(rate[v] * dt / sc[role])
-        elif role not in (ROLE_CONST, ROLE_SPIN):
+        elif role not in _UNTRAINED:
             s = sc[role]
             if rate[v] >= 0.0:
                 du[v] = math.asinh((y0[v] + rate[v] * dt) / s) - math.asinh(y0[v] / s)
@@ -282,7 +285,7 @@ def encode_raw(y0, y1, t0, t1, halo, node_par, f_baryon, unit_energy, v_suppress
             role = VAR_ROLES[v]
             if role == ROLE_ENERGY:
                 t[v] = math.asinh((y1[v] - y0[v]) / sc[role]) - du[v]
-            elif role not in (ROLE_CONST, ROLE_SPIN):
+            elif role not in _UNTRAINED:
                 t[v] = math.asinh(y1[v] / sc[role]) - math.asinh(y0[v] / sc[role]) - du[v]
     return x, t
 
@@ -308,6 +311,10 @@ def decode(y0, d, dt, halo, node_par, f_baryon, unit_energy, rate=None, negative
             continue
         if role == ROLE_SPIN:
             y1[v] = y0[v] + (0.0 if node_par[4] != 0.0 else node_par[3]) * dt
+        elif role == ROLE_FIRST_ORDER:
+            y1[v] = y0[v] + (0.0 if rate is None else rate[v]) * dt
+            if v in NON_NEGATIVE and -negative_tolerance <= y1[v] < 0.0:
+                y1[v] = 0.0
         elif role == ROLE_ENERGY:
             y1[v] = y0[v] + sc[role] * _sinh(du[v] + d[v])
         else:

@@ -104,6 +104,7 @@ typedef struct {
 typedef struct {
   double a, z, mass, r_vir, v_vir, t_dyn, t_vir, lambda;
   double r_disk, v_disk, r_sph, v_sph;
+  double r_vir_rate;   /* darkMatterHaloScale%radiusVirialGrowthRate */
 } orc_halo;
 
 static void halo_compute(const double *P, const double *np, double t,
@@ -112,10 +113,15 @@ static void halo_compute(const double *P, const double *np, double t,
   const double om = P[GAL_P_OMEGA_MATTER], ol = P[GAL_P_OMEGA_DARK_ENERGY];
   const double h0 = P[GAL_P_HUBBLE_CONSTANT];               /* km/s/Mpc */
   const double h0_gyr = h0 / C_MPC_PER_KMS_GYR;            /* 1/Gyr     */
+  const int satellite = np[GAL_NP_IS_SATELLITE] != 0.0;
   /* basic%mass solved analytically: dark_matter_only_mass/interpolate.F90:215-235 */
-  double rate = (np[GAL_NP_IS_SATELLITE] != 0.0) ? 0.0 : np[GAL_NP_MASS_RATE];
+  double rate = satellite ? 0.0 : np[GAL_NP_MASS_RATE];
   h->mass = np[GAL_NP_MASS_REFERENCE] + rate * (t - np[GAL_NP_TIME_REFERENCE]);
-  h->a = pow(om / ol, 1.0 / 3.0) * pow(sinh(1.5 * sqrt(ol) * h0_gyr * t), 2.0 / 3.0);
+  /* The virial scales use the mean density at the time the node was last
+     isolated (dark_matter_halos/scales/virial_density_contrast.F90:356-382,
+     basic%timeLastIsolated): a satellite's reference time is that time. */
+  const double t_iso = satellite ? np[GAL_NP_TIME_REFERENCE] : t;
+  h->a = pow(om / ol, 1.0 / 3.0) * pow(sinh(1.5 * sqrt(ol) * h0_gyr * t_iso), 2.0 / 3.0);
   h->z = 1.0 / h->a - 1.0;
   double a3 = h->a * h->a * h->a;
   double rho_crit0 = 3.0 * h0 * h0 / (8.0 * C_PI * C_G_INTERNAL);  /* Msun/Mpc^3 */
@@ -138,6 +144,19 @@ static void halo_compute(const double *P, const double *np, double t,
   h->v_disk = h->v_vir;
   h->r_sph = h->r_disk;
   h->v_sph = h->v_vir;
+  /* radiusVirialGrowthRate (virial_density_contrast.F90:338-354):
+     (r_vir/3) (Mdot/M - dln(rho_vir)/dt) with rho_vir = Delta_c rho_crit0 E^2;
+     dln E^2/dt = -3 H Omega_m(a), dOmega_m(a)/dt = -3 H Omega_m(a) (1 - Omega_m(a)).
+     Zero for satellites, whose scales are those of their last isolated time. */
+  if (satellite) {
+    h->r_vir_rate = 0.0;
+  } else {
+    double hubble = h0_gyr * sqrt(om / a3 + ol);
+    double dx = -3.0 * hubble * om_a * (1.0 - om_a);
+    double dln_delta = (82.0 - 78.0 * x) * dx / delta_c;
+    double dln_e2 = -3.0 * hubble * om_a;
+    h->r_vir_rate = h->r_vir / 3.0 * (rate / h->mass - (dln_delta + dln_e2));
+  }
 }
 
 /* ---------------------------------------------------------------------- */
@@ -279,6 +298,26 @@ static double brent_root(const root_ctx *rc, double a, double b, double fa, doub
   return b;
 }
 
+/* Shape integrals of the truncated beta=2/3 profile at x = r_outer / r_core:
+   S(x) = x - atan(x) (mass: M = 4 pi rho0 r_c^3 S) and
+   N(x) = (x^2 - ln(1 + x^2)) / 2 (mass-weighted mean radius <r> = r_c N / S).
+   Below x = 0.01 both differences cancel to a few digits, so their Taylor
+   series are summed instead (truncation below 1e-16 relative).             */
+static double beta_shape_mass(double x)
+{
+  if (x < 0.01) {
+    double u = x * x;
+    return x * u * (1.0 / 3.0 - u * (1.0 / 5.0 - u * (1.0 / 7.0 - u * (1.0 / 9.0))));
+  }
+  return x - atan(x);
+}
+static double beta_shape_radius(double x)
+{
+  double u = x * x;
+  if (x < 0.01) return u * u * (1.0 / 4.0 - u * (1.0 / 6.0 - u * (1.0 / 8.0 - u * (1.0 / 10.0))));
+  return 0.5 * u - 0.5 * log(1.0 + u);
+}
+
 static void hot_compute(const orc_sys *s, const orc_halo *h, const double *y, orc_hot *hh)
 {
   const double *P = s->P;
@@ -290,12 +329,12 @@ static void hot_compute(const orc_sys *s, const orc_halo *h, const double *y, or
   hh->r_outer = ro;
   hh->x_outer = ro / hh->r_core;
   double xo = hh->x_outer;
-  hh->rho0 = m / (4.0 * C_PI * hh->r_core * hh->r_core * hh->r_core * (xo - atan(xo)));
+  hh->rho0 = m / (4.0 * C_PI * hh->r_core * hh->r_core * hh->r_core * beta_shape_mass(xo));
   hh->metal_fraction = y[GAL_V_HOT_ABUNDANCES] / m;
   /* mass-weighted mean radius of the truncated profile, used for the
      constantRotation specific angular momentum normalization
      (cooling/specific_angular_momentum/constant_rotation.F90) */
-  hh->r_mean = hh->r_core * (0.5 * xo * xo - 0.5 * log(1.0 + xo * xo)) / (xo - atan(xo));
+  hh->r_mean = hh->r_core * beta_shape_radius(xo) / beta_shape_mass(xo);
   /* timeAvailable, ageFactor=0: cooling/time_available/White-Frenk.F90:144-146 */
   hh->t_avail = h->t_dyn;
   /* simpleRadius (cooling/cooling_radius/simple.F90:313-387) */
@@ -382,22 +421,90 @@ static double outflow_rate(double sfr, double velocity, double radius, double mg
   return ejective;
 }
 
+/* Outflowing gas joins the hot halo's outflowed reservoir
+   (Node_Component_Hot_Halo_Standard_Outflowing_{Mass,Ang_Mom,Abundances}_Rate,
+   hot_halo/standard/_class.F90:598-720).  The stripped fraction of
+   hotHaloOutflowStripping standard, efficiency (1 - M(<r_outer)/M(<r_vir))
+   (hot_halo/outflow_stripping/standard.F90:133-173), is zero for centrals and
+   for satellites whose outer radius lies within the virial radius; the
+   angular momentum is raised by 1/(1 - fractionLossAngularMomentum) so that
+   the infall torque applied when the gas cools again returns it. */
+static void outflowing_add(const double *P, double mass, double ang, double z, double *dydt)
+{
+  dydt[GAL_V_HOT_OUTFLOWED_MASS] += mass;
+  dydt[GAL_V_HOT_OUTFLOWED_ANGULAR_MOMENTUM] += ang / (1.0 - P[GAL_P_FRACTION_LOSS_ANGULAR_MOMENTUM]);
+  dydt[GAL_V_HOT_OUTFLOWED_ABUNDANCES] += z;
+}
+
 /* stellarFeedbackDisksDifferentialEvolution (nodes/operators/physics/stellar_feedback/disks.F90:116-206)
-   and the spheroid twin; outflowing -> outflowed reservoir with
-   fractionStripped = 0 (hot_halo/standard/_class.F90:598-626). */
-static void feedback_apply(double outflow, double mgas, double mstar, double ang, double zgas,
+   and the spheroid twin. */
+static void feedback_apply(const double *P, double outflow, double mgas, double mstar, double ang, double zgas,
                            double *dydt, int i_mgas, int i_zgas, int i_ang)
 {
   if (!(outflow > 0.0)) return;
   double mtot = mgas + mstar;
   double rate_ang = (mtot > 0.0) ? ang * (outflow / mtot) : 0.0;
   double rate_z = (mgas > 0.0) ? (zgas / mgas) * outflow : 0.0;
-  dydt[GAL_V_HOT_OUTFLOWED_MASS] += outflow;
   dydt[i_mgas] -= outflow;
-  dydt[GAL_V_HOT_OUTFLOWED_ANGULAR_MOMENTUM] += rate_ang;
   dydt[i_ang] -= rate_ang;
-  dydt[GAL_V_HOT_OUTFLOWED_ABUNDANCES] += rate_z;
   dydt[i_zgas] -= rate_z;
+  outflowing_add(P, outflow, rate_ang, rate_z, dydt);
+}
+
+/* barInstabilityDifferentialEvolution (nodes/operators/physics/bar_instability.F90:145-248)
+   with galacticDynamicsBarInstabilityEfstathiou1982
+   (galactic_dynamics/bar_instability/Efstathiou1982.F90:153-258): the disk is
+   unstable when eps = 1.18 V / sqrt(G M_disk / r_disk) falls below the
+   threshold interpolated in gas fraction; its material moves to the spheroid
+   on t_dyn ((eps_c - eps_iso)/(eps_c - eps))^2.  Returns the timescale, or
+   -1 when the disk is stable.                                               */
+static double bar_timescale(const double *P, double mg, double ms, double ang, double radius, double velocity)
+{
+  const double eps_iso = 0.6221297315, boost = 1.1800237580, dimensionless_max = 1.0e10;
+  if (ang <= 0.0) return -1.0;
+  if (velocity <= 0.0 || radius <= 0.0) return -1.0;
+  double mdisk = mg + ms;
+  double fgas = mg / mdisk;
+  double threshold = P[GAL_P_BAR_THRESHOLD_STELLAR] * (1.0 - fgas) + P[GAL_P_BAR_THRESHOLD_GASEOUS] * fgas;
+  double estimator = DBL_MAX;                  /* huge(0d0): stable */
+  if (mdisk >= 0.0) {
+    double v_self = sqrt(C_G_INTERNAL * mdisk / radius);
+    if (v_self > 0.0 && ilogb(boost * velocity) - ilogb(v_self) <= DBL_MAX_EXP) {
+      double e = boost * velocity / v_self;
+      estimator = (e > eps_iso) ? e : eps_iso;
+    }
+  }
+  if (!(estimator < threshold)) return -1.0;
+  double c_kms = C_SPEED_LIGHT / C_KILO;
+  double t_dyn = C_MPC_PER_KMS_GYR * radius / ((velocity < c_kms) ? velocity : c_kms);
+  double iso_rel = threshold - eps_iso, est_rel = threshold - estimator;
+  double dimensionless = (iso_rel > dimensionless_max * est_rel) ? dimensionless_max : (iso_rel / est_rel) * (iso_rel / est_rel);
+  double tmin = P[GAL_P_BAR_TIMESCALE_MINIMUM];
+  return ((t_dyn > tmin) ? t_dyn : tmin) * dimensionless;
+}
+
+static void bar_instability(const double *P, const orc_halo *h, const double *y, double *dydt)
+{
+  double mg = y[GAL_V_DISK_MASS_GAS], ms = y[GAL_V_DISK_MASS_STELLAR], j = y[GAL_V_DISK_ANGULAR_MOMENTUM];
+  if (j < 0.0 || h->r_disk < 0.0 || mg < 0.0) return;
+  double tau = bar_timescale(P, mg, ms, j, h->r_disk, h->v_disk);
+  if (tau < 0.0) return;
+  double rg = fmax(0.0, mg) / tau;
+  dydt[GAL_V_DISK_MASS_GAS] -= rg;
+  dydt[GAL_V_SPHEROID_MASS_GAS] += rg;
+  double rs = fmax(0.0, ms) / tau;
+  dydt[GAL_V_DISK_MASS_STELLAR] -= rs;
+  dydt[GAL_V_SPHEROID_MASS_STELLAR] += rs;
+  double rj = fmax(0.0, j) / tau;
+  dydt[GAL_V_DISK_ANGULAR_MOMENTUM] -= (1.0 - P[GAL_P_BAR_J_RETAINED_DISK]) * rj;
+  dydt[GAL_V_SPHEROID_ANGULAR_MOMENTUM] += P[GAL_P_BAR_J_RETAINED_SPHEROID] * rj;
+  double rzg = fmax(0.0, y[GAL_V_DISK_ABUNDANCES_GAS]) / tau;
+  dydt[GAL_V_DISK_ABUNDANCES_GAS] -= rzg;
+  dydt[GAL_V_SPHEROID_ABUNDANCES_GAS] += rzg;
+  double rzs = fmax(0.0, y[GAL_V_DISK_ABUNDANCES_STELLAR]) / tau;
+  dydt[GAL_V_DISK_ABUNDANCES_STELLAR] -= rzs;
+  dydt[GAL_V_SPHEROID_ABUNDANCES_STELLAR] += rzs;
+  /* the efstathiou1982 class applies no external torque (specific torque 0) */
 }
 
 /* ---------------------------------------------------------------------- */
@@ -499,6 +606,31 @@ static double hot_mode_fraction(double rc_over_rvir)
 
 typedef struct { double mdot_sph, mdot_hot, mdot, p_jet, eff_rad; } orc_bh;
 
+/* blackHoleWindCiotti2009%power (black_holes/winds/Ciotti2009.F90:113-193):
+   the wind carries efficiencyWind (times the radiative efficiency) of the
+   accreted rest-mass energy, coupled to the spheroid gas by the smoothstep of
+   P_ISM / P_wind - 1/2.  [Msun (km/s)^2 / Gyr]                              */
+static double wind_power(const double *P, const orc_bh *b, double mgas_sph, double radius_sph)
+{
+  const double velocity_wind = 1.0e4, temperature_ism = 1.0e4;   /* km/s, K */
+  double eff = P[GAL_P_BH_WIND_EFFICIENCY];
+  if (P[GAL_P_BH_WIND_SCALES_WITH_RADIATIVE] != 0.0) eff *= b->eff_rad;
+  if (b->mdot <= 0.0 || eff <= 0.0) return 0.0;
+  double coupled = 0.0;
+  if (mgas_sph > 0.0 && radius_sph > 0.0) {
+    double r = radius_sph * C_MEGAPARSEC;
+    double p_wind = eff * b->mdot * C_MASS_SOLAR / C_GIGAYEAR * C_SPEED_LIGHT * C_SPEED_LIGHT / 4.0 / C_PI
+                    / (velocity_wind * C_KILO) / (r * r);
+    double p_ism = 3.0 / 4.0 / C_PI * mgas_sph * C_MASS_SOLAR / (r * r * r) / C_MASS_H_ATOM * 3.0 / 2.0
+                   * C_BOLTZMANN * temperature_ism;
+    double x = p_ism / p_wind - 0.5;
+    if (x <= 0.0) coupled = 0.0;
+    else if (x >= 1.0) coupled = 1.0;
+    else coupled = 3.0 * x * x - 2.0 * x * x * x;
+  }
+  return eff * coupled * b->mdot * (C_SPEED_LIGHT / C_KILO) * (C_SPEED_LIGHT / C_KILO);
+}
+
 static void bh_accretion(const orc_sys *s, const orc_halo *h, const orc_hot *hh, const double *y, orc_bh *b)
 {
   const double *P = s->P;
@@ -547,6 +679,86 @@ static void bh_accretion(const orc_sys *s, const orc_halo *h, const orc_hot *hh,
 }
 
 /* ---------------------------------------------------------------------- */
+/* CGMOuterRadiusRamPressureStripping
+   (circumgalactic_medium/outer_radius/ram_pressure_stripping.F90:111-242). */
+static double hot_density(const orc_hot *hh, double r)
+{
+  return hh->valid ? beta_density(hh, r) : 0.0;
+}
+
+/* hotHaloRamPressureStrippingFont2008%radiusStripped
+   (hot_halo/ram_pressure_stripping/Font2008.F90:115-267): the radius where
+   the gravitational binding force per area, formFactor G M(<r) rho_hot(r) / r,
+   equals the ram pressure.  With the total enclosed mass of the isothermal
+   halo, M(<r) = M_vir r / r_vir (the profile behind the virial temperature of
+   the hot gas), the binding force is formFactor V_vir^2 rho_hot(r), and the
+   beta profile inverts it in closed form: r = r_c sqrt(formFactor V_vir^2
+   rho_0 / F - 1), capped at the outer radius where the truncated profile
+   ends.  The reference's bracketing decisions are kept: the virial radius if
+   the binding force there exceeds F, zero if it does not at 1e-6 r_vir.     */
+static double ram_pressure_radius(const double *P, const orc_halo *h, const orc_hot *hh, double force)
+{
+  const double rv = h->r_vir, ff = P[GAL_P_RAM_PRESSURE_FORM_FACTOR];
+  const double bind = ff * h->v_vir * h->v_vir;
+  if (bind * hot_density(hh, rv) - force >= 0.0) return rv;
+  if (bind * hot_density(hh, 1.0e-6 * rv) - force <= 0.0) return 0.0;
+  double r = hh->r_core * sqrt(bind * hh->rho0 / force - 1.0);
+  if (r > hh->r_outer) r = hh->r_outer;
+  return r;
+}
+
+/* hotHaloRamPressureTimescaleRamPressureAcceleration
+   (hot_halo/ram_pressure_stripping/timescale/ram_pressure_acceleration.F90:79-134) */
+static double ram_pressure_timescale(const orc_halo *h, const orc_hot *hh, double r_outer, double force,
+                                     double v_host)
+{
+  if (force <= 0.0) return DBL_MAX;
+  double sigma = r_outer * hot_density(hh, r_outer);
+  if (r_outer <= 0.0 || sigma <= 0.0) return h->t_dyn;
+  double a = sqrt(2.0 * r_outer * sigma / force), b = r_outer / 10.0 / v_host;
+  return C_MPC_PER_KMS_GYR * ((a > b) ? a : b);
+}
+
+static void cgm_outer_radius(const orc_sys *s, const orc_halo *h, const orc_hot *hh, const double *y,
+                             double *dydt)
+{
+  const double *P = s->P;
+  const double ro = y[GAL_V_HOT_OUTER_RADIUS], rv = h->r_vir;
+  /* haloDynamicalTimeRate (hot_halo/outflow_reincorporation/halo_dynamical_time.F90:113-128) */
+  const double m_return = y[GAL_V_HOT_OUTFLOWED_MASS] * P[GAL_P_REINCORPORATION_MULTIPLIER] / h->t_dyn;
+  if (ro < rv) {
+    double rho = hot_density(hh, ro);
+    if (ro > 0.0 && rho > 0.0) {
+      double fb = P[GAL_P_OMEGA_BARYON] / P[GAL_P_OMEGA_MATTER];
+      double rho_min = fb * h->mass / (rv * rv * rv) / 4.0 / C_PI;
+      dydt[GAL_V_HOT_OUTER_RADIUS] += m_return / 4.0 / C_PI / (ro * ro) / ((rho > rho_min) ? rho : rho_min);
+    } else if (m_return > 0.0) {
+      dydt[GAL_V_HOT_OUTER_RADIUS] += m_return / h->mass * rv;
+    }
+  }
+  if (s->np[GAL_NP_IS_SATELLITE] != 0.0) {
+    const double force = s->np[GAL_NP_RAM_PRESSURE_FORCE];
+    double r_rp = ram_pressure_radius(P, h, hh, force);
+    double dr = 0.0;
+    if (r_rp < ro && y[GAL_V_HOT_ANGULAR_MOMENTUM] > 0.0)
+      dr = (r_rp - ro) / ram_pressure_timescale(h, hh, ro, force, s->np[GAL_NP_HOST_VELOCITY_VIRIAL]);
+    if (dr != 0.0 && y[GAL_V_HOT_MASS] > 0.0 && ro > 1.0e-3 * rv) {
+      double m = y[GAL_V_HOT_MASS];
+      double loss = 4.0 * C_PI * hot_density(hh, ro) * ro * ro * dr;
+      dydt[GAL_V_HOT_OUTER_RADIUS] += dr;
+      dydt[GAL_V_HOT_MASS] += loss;
+      dydt[GAL_V_HOT_ANGULAR_MOMENTUM] += y[GAL_V_HOT_ANGULAR_MOMENTUM] * (loss / m);
+      dydt[GAL_V_HOT_ABUNDANCES] += y[GAL_V_HOT_ABUNDANCES] * (loss / m);
+      /* hotHaloOutflowStripping standard: satellites are stripped */
+      dydt[GAL_V_HOT_STRIPPED_MASS] -= loss;
+      dydt[GAL_V_HOT_STRIPPED_ABUNDANCES] += y[GAL_V_HOT_ABUNDANCES] * (-loss / m);
+    }
+  } else {
+    dydt[GAL_V_HOT_OUTER_RADIUS] += h->r_vir_rate;
+  }
+}
+
+/* ---------------------------------------------------------------------- */
 /* The RHS: nodeOperatorMulti%differentialEvolution over the operators of
    parameters/quickTest.xml:220-289 in list order. */
 static int rhs_eval(orc_sys *s, double t, const double *y, double *dydt)
@@ -566,6 +778,13 @@ static int rhs_eval(orc_sys *s, double t, const double *y, double *dydt)
   halo_compute(P, s->np, t, y, &h);
   /* spin%angularMomentum growth (halo_angular_momentum_interpolate.F90) */
   dydt[GAL_V_SPIN_ANGULAR_MOMENTUM] += (s->np[GAL_NP_IS_SATELLITE] != 0.0) ? 0.0 : s->np[GAL_NP_SPIN_RATE];
+  const int ops = (mode == GAL_PHYSICS_STANDARD) ? (int)P[GAL_P_OPERATORS] : 0;
+  /* satelliteMassLoss (nodes/operators/physics/satellite/mass_loss/_class.F90:176-200):
+     the bound mass of an isolated halo follows its basic mass; a satellite's
+     loses mass at darkMatterHaloMassLossRate, whose default (zero,
+     dark_matter_halos/mass_loss_rates/_class.F90:39) quickTest.xml keeps. */
+  if (ops & GAL_OPERATOR_SATELLITE_MASS_LOSS)
+    dydt[GAL_V_SATELLITE_BOUND_MASS] += (s->np[GAL_NP_IS_SATELLITE] != 0.0) ? 0.0 : s->np[GAL_NP_MASS_RATE];
 
   /* starFormationDisks: nodes/operators/physics/star_formation/disks.F90:200-325 */
   double mg_d = y[GAL_V_DISK_MASS_GAS], ms_d = y[GAL_V_DISK_MASS_STELLAR];
@@ -583,10 +802,11 @@ static int rhs_eval(orc_sys *s, double t, const double *y, double *dydt)
   /* starFormationSpheroids: nodes/operators/physics/star_formation/spheroids.F90:152 */
   double mg_s = y[GAL_V_SPHEROID_MASS_GAS], ms_s = y[GAL_V_SPHEROID_MASS_STELLAR];
   double j_s = y[GAL_V_SPHEROID_ANGULAR_MOMENTUM], zg_s = y[GAL_V_SPHEROID_ABUNDANCES_GAS];
-  int sph_ok = (mode == GAL_PHYSICS_STANDARD) && !(j_s < 0.0 || h.r_sph < 0.0 || mg_s < 0.0);
+  int sph_ok = !(j_s < 0.0 || h.r_sph < 0.0 || mg_s < 0.0);
   double sfr_s = 0.0;
   if (sph_ok) {
-    double tau = sf_timescale(h.v_sph, h.r_sph, P[GAL_P_SF_SPHEROID_EFFICIENCY],
+    double tau = (mode == GAL_PHYSICS_BOX) ? P[GAL_P_BOX_TIMESCALE]
+               : sf_timescale(h.v_sph, h.r_sph, P[GAL_P_SF_SPHEROID_EFFICIENCY],
                               P[GAL_P_SF_SPHEROID_EXPONENT_VELOCITY], P[GAL_P_SF_SPHEROID_TIMESCALE_MINIMUM]);
     sfr_s = (tau > 0.0) ? mg_s / tau : 0.0;
     stellar_rates(P, sfr_s, mg_s, zg_s, dydt, GAL_V_SPHEROID_MASS_STELLAR, GAL_V_SPHEROID_MASS_STELLAR_FORMED,
@@ -596,15 +816,17 @@ static int rhs_eval(orc_sys *s, double t, const double *y, double *dydt)
   if (disk_ok) {
     double out = outflow_rate(sfr_d, h.v_disk, h.r_disk, mg_d, P[GAL_P_FB_DISK_VELOCITY], P[GAL_P_FB_DISK_EXPONENT],
                               P[GAL_P_FB_DISK_TIMESCALE_MINIMUM], mode == GAL_PHYSICS_BOX, P[GAL_P_BOX_OUTFLOW_FRACTION]);
-    feedback_apply(out, mg_d, ms_d, j_d, zg_d, dydt, GAL_V_DISK_MASS_GAS, GAL_V_DISK_ABUNDANCES_GAS, GAL_V_DISK_ANGULAR_MOMENTUM);
+    feedback_apply(P, out, mg_d, ms_d, j_d, zg_d, dydt, GAL_V_DISK_MASS_GAS, GAL_V_DISK_ABUNDANCES_GAS, GAL_V_DISK_ANGULAR_MOMENTUM);
   }
-  if (mode == GAL_PHYSICS_BOX) return ST_SUCCESS;
   /* stellarFeedbackSpheroids: nodes/operators/physics/stellar_feedback/spheroids.F90:116 */
   if (sph_ok) {
     double out = outflow_rate(sfr_s, h.v_sph, h.r_sph, mg_s, P[GAL_P_FB_SPHEROID_VELOCITY], P[GAL_P_FB_SPHEROID_EXPONENT],
-                              P[GAL_P_FB_SPHEROID_TIMESCALE_MINIMUM], 0, 0.0);
-    feedback_apply(out, mg_s, ms_s, j_s, zg_s, dydt, GAL_V_SPHEROID_MASS_GAS, GAL_V_SPHEROID_ABUNDANCES_GAS, GAL_V_SPHEROID_ANGULAR_MOMENTUM);
+                              P[GAL_P_FB_SPHEROID_TIMESCALE_MINIMUM], mode == GAL_PHYSICS_BOX,
+                              P[GAL_P_BOX_OUTFLOW_FRACTION_SPHEROID]);
+    feedback_apply(P, out, mg_s, ms_s, j_s, zg_s, dydt, GAL_V_SPHEROID_MASS_GAS, GAL_V_SPHEROID_ABUNDANCES_GAS, GAL_V_SPHEROID_ANGULAR_MOMENTUM);
   }
+  if (mode == GAL_PHYSICS_BOX) return ST_SUCCESS;
+  if (ops & GAL_OPERATOR_BAR_INSTABILITY) bar_instability(P, &h, y, dydt);
   orc_hot hh;
   hot_compute(s, &h, y, &hh);
   /* blackHolesAccretion: nodes/operators/physics/black_holes/accretion.F90:113-170 */
@@ -631,6 +853,23 @@ static int rhs_eval(orc_sys *s, double t, const double *y, double *dydt)
         dydt[GAL_V_HOT_ANGULAR_MOMENTUM] -= b.mdot_hot * (y[GAL_V_HOT_ANGULAR_MOMENTUM] / mh);
       }
       dydt[GAL_V_BH_SPIN] += spin_up;
+    }
+  }
+  /* blackHolesWinds (nodes/operators/physics/black_holes/winds.F90:102-133):
+     the wind power enters the spheroid gas as energy, which drives an outflow
+     efficiencyEnergeticOutflow P / V_spheroid^2 into the hot halo
+     (Node_Component_Spheroid_Standard_Energy_Gas_Input_Rate,
+     spheroid/standard/_class.F90:673-725). */
+  if (ops & GAL_OPERATOR_BLACK_HOLE_WINDS) {
+    double power = wind_power(P, &b, mg_s, h.r_sph);
+    if (power > 0.0 && mg_s > 0.0 && mg_s + ms_s > 0.0 && h.v_sph > 0.0) {
+      double mo = P[GAL_P_SPHEROID_ENERGETIC_OUTFLOW] * power / (h.v_sph * h.v_sph);
+      double jo = (mo / (mg_s + ms_s)) * j_s;
+      double zo = (mo / mg_s) * zg_s;
+      dydt[GAL_V_SPHEROID_MASS_GAS] -= mo;
+      dydt[GAL_V_SPHEROID_ANGULAR_MOMENTUM] -= jo;
+      dydt[GAL_V_SPHEROID_ABUNDANCES_GAS] -= zo;
+      outflowing_add(P, mo, jo, zo, dydt);
     }
   }
   /* CGMAccretion (nodes/operators/physics/circumgalactic_medium/accretion.F90:517)
@@ -673,18 +912,23 @@ static int rhs_eval(orc_sys *s, double t, const double *y, double *dydt)
     double cool = cooling_rate_wf(s, &h, &hh, y);
     double heat = P[GAL_P_EFFICIENCY_RADIO_MODE] * b.p_jet / (h.v_vir * h.v_vir);
     if (heat > cool) {
-      double out = heat - cool;
-      double cap = P[GAL_P_RATE_MAXIMUM_EXPULSION] * mh / h.t_dyn;
-      if (cap < out) out = cap;
-      double rz = y[GAL_V_HOT_ABUNDANCES] * (out / mh);
-      double rj = y[GAL_V_HOT_ANGULAR_MOMENTUM] * (out / mh);
-      dydt[GAL_V_HOT_MASS] -= out;
-      dydt[GAL_V_HOT_ABUNDANCES] -= rz;
-      dydt[GAL_V_HOT_ANGULAR_MOMENTUM] -= rj;
-      /* hotHaloMassEjection: the ejected gas joins the outflowed reservoir */
-      dydt[GAL_V_HOT_OUTFLOWED_MASS] += out;
-      dydt[GAL_V_HOT_OUTFLOWED_ABUNDANCES] += rz;
-      dydt[GAL_V_HOT_OUTFLOWED_ANGULAR_MOMENTUM] += rj;
+      if (P[GAL_P_EXCESS_HEAT_DRIVES_OUTFLOW] != 0.0) {
+        double out = heat - cool;
+        double cap = P[GAL_P_RATE_MAXIMUM_EXPULSION] * mh / h.t_dyn;
+        if (cap < out) out = cap;
+        double rz = y[GAL_V_HOT_ABUNDANCES] * (out / mh);
+        double rj = y[GAL_V_HOT_ANGULAR_MOMENTUM] * (out / mh);
+        dydt[GAL_V_HOT_MASS] -= out;
+        dydt[GAL_V_HOT_ABUNDANCES] -= rz;
+        dydt[GAL_V_HOT_ANGULAR_MOMENTUM] -= rj;
+        /* the expelled gas leaves the node; a satellite's joins its stripped
+           reservoir (hotHaloOutflowStripping standard never strips centrals,
+           outflow_stripping/standard.F90:120-131) */
+        if (s->np[GAL_NP_IS_SATELLITE] != 0.0) {
+          dydt[GAL_V_HOT_STRIPPED_MASS] += out;
+          dydt[GAL_V_HOT_STRIPPED_ABUNDANCES] += rz;
+        }
+      }
     } else if (cool > heat) {
       cool = cool - heat;
       if (cool < 0.0) cool = 0.0;
@@ -700,6 +944,7 @@ static int rhs_eval(orc_sys *s, double t, const double *y, double *dydt)
       dydt[GAL_V_DISK_ANGULAR_MOMENTUM] += rj * (1.0 - P[GAL_P_FRACTION_LOSS_ANGULAR_MOMENTUM]);
     }
   }
+  if (ops & GAL_OPERATOR_CGM_OUTER_RADIUS) cgm_outer_radius(s, &h, &hh, y, dydt);
   /* coolingEnergyRadiated (nodes/operators/physics/cooling_energy_radiated.F90:200-260) */
   if (hh.valid) {
     double ro = y[GAL_V_HOT_OUTER_RADIUS];
@@ -824,12 +1069,15 @@ static int post_step(orc_sys *s, double t, double *y)
   if (mode == GAL_PHYSICS_TEST) return status;
   orc_halo h;
   halo_compute(s->P, s->np, t, y, &h);
+  /* postStepZeroNegativeMasses (disk/standard/_class.F90, spheroid/standard/_class.F90) */
+  if (s->P[GAL_P_POST_STEP_ZERO_NEGATIVE_MASSES] != 0.0) {
   component_post_step(y, GAL_V_DISK_MASS_STELLAR, GAL_V_DISK_ABUNDANCES_STELLAR, GAL_V_DISK_MASS_GAS,
                       GAL_V_DISK_ABUNDANCES_GAS, GAL_V_DISK_ANGULAR_MOMENTUM, h.r_disk, h.v_disk,
                       y[GAL_V_SPIN_ANGULAR_MOMENTUM], h.mass, &status);
   component_post_step(y, GAL_V_SPHEROID_MASS_STELLAR, GAL_V_SPHEROID_ABUNDANCES_STELLAR, GAL_V_SPHEROID_MASS_GAS,
                       GAL_V_SPHEROID_ABUNDANCES_GAS, GAL_V_SPHEROID_ANGULAR_MOMENTUM, h.r_sph, h.v_sph,
                       y[GAL_V_SPIN_ANGULAR_MOMENTUM], h.mass, &status);
+  }
   /* hot_halo/standard/_class.F90:468-475 */
   if (y[GAL_V_HOT_MASS] < 0.0) { y[GAL_V_HOT_MASS] = 0.0; if (status == ST_SUCCESS) status = ST_CONTINUE; }
   if (y[GAL_V_HOT_OUTER_RADIUS] < 0.0) { y[GAL_V_HOT_OUTER_RADIUS] = 0.0; if (status == ST_SUCCESS) status = ST_CONTINUE; }

@@ -105,7 +105,7 @@ def _walk_tree(parent0, child0, sib0, mass, time0, cosmo, w, lo, hi):
             rp, vp = cosmo.virial(mp, tp)
             mr = (mp - m) / (tp - t)
             jr = (lam * mp * vp * rp - j) / (tp - t)
-        par[i] = [t, m, mr, jr, 0.0]
+        par[i] = [t, m, mr, jr, 0.0, 0.0, 0.0]
 
     def halo_mass(i, t):
         q = par[i]
@@ -257,7 +257,7 @@ def _walk_tree(parent0, child0, sib0, mass, time0, cosmo, w, lo, hi):
                             hand_over_satellites(i, p)
                             is_sat[i] = True
                             tm[i] = tmerge
-                            par[i] = [T[i], halo_mass(i, T[i]), 0.0, 0.0, 1.0]
+                            par[i] = [T[i], halo_mass(i, T[i]), 0.0, 0.0, 1.0, 0.0, 0.0]
                             log.append((SATELLITE, i, p, T[i], tmerge, None))
                     elif S[i] < 0:
                         log.append((PROMOTE, i, p, T[i], T[p], None))

@@ -1,0 +1,154 @@
+"""GPU parity of the quickTest node operators added to the RHS and of the
+node-evolver's failure branches, against the CPU oracle.
+
+* RHS (nodeOperatorMulti over parameters/quickTest.xml:220-289): isolated
+  halos and satellites under ram pressure, with every subset of the optional
+  operators (satelliteMassLoss, barInstability, blackHolesWinds,
+  CGMOuterRadiusRamPressureStripping) and with excess-heat expulsion on and
+  off: |f_gpu - f_oracle| <= 1e-9 (|f_oracle| + scale).
+* Evolution of satellites stripped by ram pressure: every node within 10x the
+  ODE tolerance of the oracle, identical statuses.
+* The trial loop of node_evolver/standard.F90:587-718:
+  - the reference's non-negativity model (testSuite/parameters/enforceNonNegativity/
+    rapidDepletion*.xml) at tolerances 0.1 to 1e-9 with and without
+    enforceNonNegativity: no negative spheroid masses when enforcing (the
+    reference's assertion, testSuite/test-enforceNonNegativity.py:74-83), and
+    the same statuses, attempt and step counts and states as the oracle;
+  - a reservoir that stays negative: every step is forbidden, the step size
+    underflows and the non-negativity rescue zeroes it and succeeds
+    (standard.F90:657-684);
+  - tolerances that cannot be met: all eight trials at h/2^k fail,
+    errorStatusUnderflow (15), the state restored (standard.F90:694-718), with
+    the oracle's attempt counts (which depend on each trial's first step).
+"""
+import numpy as np
+import pytest
+
+import oracle
+from galacticus_amd import params as gp
+from galacticus_amd import workloads as wl
+
+pytestmark = pytest.mark.gpu
+
+V = wl.V
+
+
+@pytest.fixture(scope="module")
+def torch_cuda():
+    import torch
+    assert torch.cuda.is_available(), "GPU tests need a GPU"
+    return torch
+
+
+def _evolver(n, physics, table, **kw):
+    from galacticus_amd.evolver import NodeEvolverBatched
+    return NodeEvolverBatched(n, physics, table, **kw)
+
+
+@pytest.mark.parametrize("workload", ["isolated", "satellites"])
+@pytest.mark.parametrize("operators,heat", [(0xF, 1.0), (0xF, 0.0), (0x0, 1.0), (0x1, 1.0), (0x2, 1.0), (0x4, 1.0),
+                                            (0x8, 1.0)])
+def test_rhs_operators_match_oracle(torch_cuda, cooling_table, workload, operators, heat):
+    torch = torch_cuda
+    n = 1024
+    b = wl.isolated_halos(n, seed=13) if workload == "isolated" else wl.satellite_halos(n, seed=13)
+    p = gp.physics_params(nodeOperators=operators, **{"CGMCoolingHeating.excessHeatDrivesOutflow": heat})
+    print(f"rhs {workload} operators={operators:#x} heat={heat}", flush=True)
+    ev = _evolver(n, p, cooling_table)
+    rng = np.random.default_rng(5)
+    t = b.t_start + rng.uniform(0, 1, n) * (b.t_end - b.t_start)
+    dev = lambda a: torch.from_numpy(np.ascontiguousarray(a)).cuda()
+    fg = ev.rhs_device(dev(t), dev(b.y), dev(b.node_par)).cpu().numpy()
+    fo = oracle.rhs(t, b.y, b.node_par, p, cooling_table, root_tol_rel=1e-15)
+    sc = oracle.scales(b.t_start, b.y, b.node_par, p)
+    bad = np.abs(fg - fo) > 1e-9 * (np.abs(fo) + sc)
+    assert not bad.any(), [(wl.VAR_NAMES[v], int(np.sum(bad[v]))) for v in range(30) if bad[v].any()]
+
+
+def test_satellite_stripping_evolution_matches_oracle(torch_cuda, cooling_table):
+    n = 1000
+    b = wl.satellite_halos(n, seed=17)
+    t_end = b.t_start + 0.25 * (b.t_end - b.t_start)
+    p = gp.physics_params()
+    print(f"satellite evolve n={n}", flush=True)
+    ev = _evolver(n, p, cooling_table, odeToleranceAbsolute=1e-6, odeToleranceRelative=1e-3)
+    yg, hg, scg, stg, cntg = ev.evolve(b.y, b.node_par, b.t_start, t_end, b.h_step)
+    yo, ho, sco, sto, cnto = oracle.evolve(b.y, b.node_par, b.t_start, t_end, b.h_step, p, cooling_table, 1e-6, 1e-3,
+                                           gp.FLAG_REUSE_STEP_SIZE, root_tol_rel=1e-15, nthreads=8)
+    assert np.array_equal(stg, sto)
+    assert (stg == 0).all()
+    stripped = yo[V["HOT_STRIPPED_MASS"]] > 0.0
+    print("stripped satellites", int(stripped.sum()), "attempts max", int(cntg[0].max()), flush=True)
+    assert stripped.sum() > 50
+    tol = 10.0 * (1e-3 * np.abs(yo) + 1e-6 * sco)
+    bad = np.abs(yg - yo) > tol
+    assert not bad.any(), [(wl.VAR_NAMES[v], int(np.sum(bad[v]))) for v in range(30) if bad[v].any()]
+    assert np.mean(cntg[0] == cnto[0]) > 0.95
+
+
+@pytest.mark.parametrize("tol", [0.1, 0.01, 1e-3, 1e-4, 1e-6, 1e-9])
+@pytest.mark.parametrize("enforce", [False, True])
+def test_rapid_depletion_non_negativity(torch_cuda, cooling_table, tol, enforce):
+    p, b = wl.rapid_depletion()
+    print(f"rapidDepletion tol={tol} enforce={enforce}", flush=True)
+    ev = _evolver(1, p, cooling_table, odeToleranceAbsolute=tol, odeToleranceRelative=tol,
+                  enforceNonNegativity=enforce)
+    yg, hg, scg, stg, cntg = ev.evolve(b.y, b.node_par, b.t_start, b.t_end, b.h_step)
+    flags = gp.FLAG_REUSE_STEP_SIZE | (gp.FLAG_ENFORCE_NON_NEGATIVITY if enforce else 0)
+    yo, ho, sco, sto, cnto = oracle.evolve(b.y, b.node_par, b.t_start, b.t_end, b.h_step, p, cooling_table, tol, tol,
+                                           flags)
+    assert np.array_equal(stg, sto) and stg[0] == 0
+    assert np.array_equal(cntg[:2], cnto[:2])
+    np.testing.assert_allclose(yg, yo, rtol=1e-10, atol=1e-6)
+    spheroid = [V[k] for k in ("SPHEROID_MASS_GAS", "SPHEROID_MASS_STELLAR", "SPHEROID_ABUNDANCES_GAS",
+                               "SPHEROID_ABUNDANCES_STELLAR")]
+    if enforce:
+        assert (yg[spheroid] >= 0.0).all()
+    elif tol >= 0.01:
+        # without enforcement the loose tolerances overshoot below zero
+        assert (yg[spheroid] < 0.0).any()
+
+
+def test_non_negativity_rescue(torch_cuda, cooling_table):
+    """A non-negative reservoir with zero rate that starts negative forbids
+    every step: with enforceNonNegativity the solve fails, the negative values
+    are set to zero and the node succeeds with the trial's step size."""
+    p, b = wl.rapid_depletion()
+    y0 = np.repeat(b.y, 4, axis=1)
+    y0[V["HOT_UNACCRETED_MASS"]] = -1.0e3
+    npar = np.repeat(b.node_par, 4, axis=1)
+    t0, t1 = np.repeat(b.t_start, 4), np.repeat(b.t_end, 4)
+    h0 = np.array([-1.0, 0.05, 0.3, 2.0])
+    ev = _evolver(4, p, cooling_table, odeToleranceAbsolute=1e-3, odeToleranceRelative=1e-3,
+                  enforceNonNegativity=True)
+    yg, hg, _, stg, cntg = ev.evolve(y0, npar, t0, t1, h0)
+    yo, ho, _, sto, cnto = oracle.evolve(y0, npar, t0, t1, h0, p, cooling_table, 1e-3, 1e-3,
+                                         gp.FLAG_REUSE_STEP_SIZE | gp.FLAG_ENFORCE_NON_NEGATIVITY)
+    assert (sto == 0).all() and np.array_equal(stg, sto)
+    assert (cnto[1] == 0).all()                    # no step was ever accepted
+    assert np.array_equal(cntg[:2], cnto[:2])
+    assert np.array_equal(hg, ho) and np.array_equal(hg, h0)
+    expect = y0.copy()
+    expect[V["HOT_UNACCRETED_MASS"]] = 0.0
+    assert np.array_equal(yg, expect) and np.array_equal(yo, expect)
+
+
+@pytest.mark.parametrize("reuse", [False, True])
+def test_impossible_tolerance_underflow(torch_cuda, cooling_table, reuse):
+    n = 64
+    b = wl.isolated_halos(n, seed=6)
+    p = gp.physics_params()
+    h0 = -np.ones(n) if not reuse else (b.t_end - b.t_start) * np.linspace(0.01, 1.5, n)
+    print(f"underflow reuse={reuse}", flush=True)
+    ev = _evolver(n, p, cooling_table, odeToleranceAbsolute=1e-300, odeToleranceRelative=1e-300,
+                  reuseODEStepSize=reuse)
+    yg, hg, _, stg, cntg = ev.evolve(b.y, b.node_par, b.t_start, b.t_end, h0)
+    flags = gp.FLAG_REUSE_STEP_SIZE if reuse else 0
+    yo, ho, _, sto, cnto = oracle.evolve(b.y, b.node_par, b.t_start, b.t_end, h0, p, cooling_table, 1e-300, 1e-300,
+                                         flags)
+    assert (sto == 15).all() and np.array_equal(stg, sto)
+    assert np.array_equal(yg, b.y) and np.array_equal(yo, b.y)
+    assert np.array_equal(cntg[:2], cnto[:2])
+    assert (cntg[1] == 0).all()
+    print("attempts per node over eight trials: min", int(cntg[0].min()), "max", int(cntg[0].max()), flush=True)
+

@@ -137,7 +137,7 @@ def test_ode_solver_closed_forms(torch_cuda, cooling_table):
         j = len(xs) + i
         y[1, j], y[2, j] = 1.0, 0.0      # oscillator
         t0[j], t1[j] = 0.0, x
-    npar = np.zeros((5, n))
+    npar = np.zeros((gp.NPARAM_NODE, n))
     sc = np.ones((30, n))
     ev = _evolver(n, p, table=cooling_table, odeToleranceAbsolute=1e-9, odeToleranceRelative=1e-9,
                   reuseODEStepSize=False, scales_provided=True)

@@ -273,9 +273,15 @@ def test_scales_follow_component_rules():
 
 # --------------------------------------------------------- whole RHS -------
 def test_baryon_and_metal_budgets(table):
-    p = gp.physics_params()
+    # Gas expelled by excess heating leaves the node (cooling_heating.F90:216-240),
+    # so the budget closes with that outflow off, as in the reference's own
+    # mass-conservation model (testSuite/parameters/test-mass-conservation-standard.xml
+    # has no circumgalactic heating).  At a tight tolerance the post-step zeroing
+    # of negative reservoirs (which the bar instability's fast transfers trigger)
+    # adds mass only at the 1e-10 level.
+    p = gp.physics_params(**{"CGMCoolingHeating.excessHeatDrivesOutflow": 0.0})
     b = wl.isolated_halos(300, seed=9)
-    y, h, sc, st, cnt = oracle.evolve(b.y, b.node_par, b.t_start, b.t_end, b.h_step, p, table, 1e-6, 1e-4,
+    y, h, sc, st, cnt = oracle.evolve(b.y, b.node_par, b.t_start, b.t_end, b.h_step, p, table, 1e-10, 1e-8,
                                       gp.FLAG_REUSE_STEP_SIZE, nthreads=4)
     assert (st == 0).all()
     fb = p[gp.P["OmegaBaryon"]] / p[gp.P["OmegaMatter"]]
@@ -315,10 +321,16 @@ def test_rhs_no_sources_without_gas(table):
     npar[2] = 0.0
     npar[3] = 0.0
     f = oracle.rhs(b.t_start, y, npar, p, table)
-    assert np.all(f[V["DISK_MASS_STELLAR"]] == 0.0)
-    assert np.all(f[V["SPHEROID_MASS_STELLAR"]] == 0.0)
+    # no star formation; a bar instability may still move stars from the disk
+    # to the spheroid, conserving them
+    assert np.all(f[V["DISK_MASS_STELLAR_FORMED"]] == 0.0)
+    assert np.all(f[V["SPHEROID_MASS_STELLAR_FORMED"]] == 0.0)
+    assert np.all(f[V["DISK_MASS_STELLAR"]] + f[V["SPHEROID_MASS_STELLAR"]] == 0.0)
     assert np.all(f[V["BH_MASS"]] == 0.0)
     assert np.all(f[V["HOT_MASS"]] == 0.0)
+    f = oracle.rhs(b.t_start, y, npar, gp.physics_params(nodeOperators=0), table)
+    assert np.all(f[V["DISK_MASS_STELLAR"]] == 0.0)
+    assert np.all(f[V["SPHEROID_MASS_STELLAR"]] == 0.0)
 
 
 def test_isco_schwarzschild():
@@ -376,9 +388,11 @@ def test_black_hole_spin_near_zero_stays_finite(cooling_table):
     """Regression: for spins of order 1e-10 the ISCO factor A1 rounds to just
     above 3 and sqrt(3 - A1) turned NaN.  A leaf halo of a tree (hot gas only,
     a 100 Msun black hole with zero spin) spins its black hole up through that
-    range in its first interval."""
+    range in its first interval (with the outer radius held fixed: when it
+    follows the growing virial radius the gas cools at r_outer = r_vir and the
+    hot-mode fraction, hence the accretion, is zero)."""
     from galacticus_amd import params as gp, trees as tr, workloads as wl
-    p = gp.physics_params()
+    p = gp.physics_params(nodeOperators=0x7)
     cosmo = tr.cosmology(p)
     t0 = float(wl.scale_factor_to_cosmic_time(1.0))
     t_z5 = float(wl.scale_factor_to_cosmic_time(1.0 / 6.0))

@@ -109,9 +109,10 @@ def test_encoding_roles_match_state_layout():
     assert om.INPUT_VARS == INPUT_VARS
     assert om.VAR_ROLES[V["HOT_ENERGY_RADIATED"]] == om.ROLE_ENERGY
     assert om.VAR_ROLES[V["SPIN_ANGULAR_MOMENTUM"]] == om.ROLE_SPIN
-    for k in ("HOT_OUTER_RADIUS", "HOT_STRIPPED_MASS", "HOT_STRIPPED_ABUNDANCES", "DISK_FRACTION_MASS_RETAINED",
-              "BH_RADIAL_POSITION", "SATELLITE_BOUND_MASS"):
+    for k in ("DISK_FRACTION_MASS_RETAINED", "BH_RADIAL_POSITION"):
         assert om.VAR_ROLES[V[k]] == om.ROLE_CONST, k
+    for k in ("HOT_OUTER_RADIUS", "HOT_STRIPPED_MASS", "HOT_STRIPPED_ABUNDANCES", "SATELLITE_BOUND_MASS"):
+        assert om.VAR_ROLES[V[k]] == om.ROLE_FIRST_ORDER, k
     for k in ("DISK_MASS_STELLAR", "SPHEROID_MASS_GAS", "HOT_UNACCRETED_MASS"):
         assert om.VAR_ROLES[V[k]] == om.ROLE_MASS
     for k in ("DISK_ABUNDANCES_GAS", "HOT_OUTFLOWED_ABUNDANCES"):
@@ -130,12 +131,14 @@ def test_encode_decode_round_trip():
         y0 = b.y[:, i]
         y1 = y0 * rng.uniform(0.5, 2.0, 30)
         y1[om.VAR_ROLES.index(om.ROLE_SPIN)] = y0[28] + b.node_par[3, i] * (b.t_end[i] - b.t_start[i])
+        rate = rng.normal(0, 1, 30) * np.abs(y0) * 0.1
         for v, r in enumerate(om.VAR_ROLES):
             if r == om.ROLE_CONST:
                 y1[v] = y0[v]
+            elif r == om.ROLE_FIRST_ORDER:
+                y1[v] = y0[v] + rate[v] * (b.t_end[i] - b.t_start[i])
         r_vir, v_vir = wl.virial(b.node_par[1, i], b.t_start[i])
         halo = dict(mass=b.node_par[1, i], r_vir=float(r_vir), v_vir=float(v_vir), t_dyn=1.0, a=0.5)
-        rate = rng.normal(0, 1, 30) * np.abs(y0) * 0.1
         _, t = om.encode_raw(y0, y1, b.t_start[i], b.t_end[i], halo, b.node_par[:, i], fb, 1.0, 30.0, 1e-3, rate)
         back = om.decode(y0, t, b.t_end[i] - b.t_start[i], halo, b.node_par[:, i], fb, 1.0, rate)
         assert np.allclose(back, y1, rtol=1e-12, atol=0), i
