@@ -251,18 +251,25 @@ __device__ __forceinline__ void rhs(const double* __restrict__ P, const DerivedC
   isco(spin, e_isco, l_isco);
   const double m_hot = y[GAL_V_HOT_MASS], r_out = y[GAL_V_HOT_OUTER_RADIUS];
   const bool hot_ok = m_hot > 0.0 && r_out > 0.0;
-  const double inv_m_hot = hot_ok ? rcp(m_hot) : 0.0;
+  // Reciprocal masses for the ratios of a reservoir (metallicity, specific
+  // angular momentum).  Below kRatioMassMin the ratio is taken as zero: rcp()
+  // of a subnormal mass overflows (NaN after the Newton steps), and a
+  // reservoir that enforceNonNegativity lets decay towards zero by halved
+  // steps reaches subnormal values; every flow such a ratio multiplies scales
+  // with the reservoir itself, so the flows dropped are below 1e-100 Msun/Gyr.
+  constexpr double kRatioMassMin = 1.0e-100;
+  const double inv_m_hot = (hot_ok && m_hot > kRatioMassMin) ? rcp(m_hot) : 0.0;
   const double zf = fmin(fmax(y[GAL_V_HOT_ABUNDANCES] * inv_m_hot, 0.0), 1.0);
   const double lz = log(clamp_metallicity(cd, zf * (1.0 / kMetallicitySolar)));
   const double mg_d = y[GAL_V_DISK_MASS_GAS], ms_d = y[GAL_V_DISK_MASS_STELLAR];
   const double j_d = y[GAL_V_DISK_ANGULAR_MOMENTUM], zg_d = y[GAL_V_DISK_ABUNDANCES_GAS];
   const double mg_s = y[GAL_V_SPHEROID_MASS_GAS], ms_s = y[GAL_V_SPHEROID_MASS_STELLAR];
   const double j_s = y[GAL_V_SPHEROID_ANGULAR_MOMENTUM], zg_s = y[GAL_V_SPHEROID_ABUNDANCES_GAS];
-  const double inv_mg_d = mg_d > 0.0 ? rcp(mg_d) : 0.0;
-  const double inv_mg_s = mg_s > 0.0 ? rcp(mg_s) : 0.0;
+  const double inv_mg_d = mg_d > kRatioMassMin ? rcp(mg_d) : 0.0;
+  const double inv_mg_s = mg_s > kRatioMassMin ? rcp(mg_s) : 0.0;
   const double mt_d = mg_d + ms_d, mt_s = mg_s + ms_s;
-  const double inv_mt_d = mt_d > 0.0 ? rcp(mt_d) : 0.0;
-  const double inv_mt_s = mt_s > 0.0 ? rcp(mt_s) : 0.0;
+  const double inv_mt_d = mt_d > kRatioMassMin ? rcp(mt_d) : 0.0;
+  const double inv_mt_s = mt_s > kRatioMassMin ? rcp(mt_s) : 0.0;
 
   // ---- halo analytics ----------------------------------------------------
   const Halo H = halo(P, D, np, t, y[GAL_V_SPIN_ANGULAR_MOMENTUM]);
@@ -432,7 +439,7 @@ __device__ __forceinline__ void rhs(const double* __restrict__ P, const DerivedC
                                   : xo - atan(xo);
   const double shape_r = small_xo ? uo * uo * (1.0 / 4.0 - uo * (1.0 / 6.0 - uo * (1.0 / 8.0 - uo * (1.0 / 10.0))))
                                   : 0.5 * uo - 0.5 * log1p(uo);
-  const double inv_shape = rcp(shape_m);
+  const double inv_shape = hot_ok ? rcp(shape_m) : 0.0;
   const double rho0 = m_hot * inv_shape * (1.0 / (4.0 * kPi)) * inv_r_core * inv_r_core * inv_r_core;
   const double r_mean = r_core * shape_r * inv_shape;
   // composition of metallicity-only abundances, fully ionized H and He
@@ -601,7 +608,7 @@ __device__ __forceinline__ void rhs(const double* __restrict__ P, const DerivedC
     const double z_hot = y[GAL_V_HOT_ABUNDANCES] * inv_m_hot;
     const double oz = y[GAL_V_HOT_ABUNDANCES] * (out * inv_m_hot);
     const double oj = y[GAL_V_HOT_ANGULAR_MOMENTUM] * (out * inv_m_hot);
-    const double cj = c * (j_hot * r_cool * rcp(r_mean));
+    const double cj = cooling ? c * (j_hot * r_cool * rcp(r_mean)) : 0.0;
     const double cz = c * z_hot;
     f[GAL_V_HOT_MASS] -= heating ? out : c;
     f[GAL_V_HOT_ABUNDANCES] -= heating ? oz : cz;
@@ -630,12 +637,16 @@ __device__ __forceinline__ void rhs(const double* __restrict__ P, const DerivedC
     const double ret_a = m_return * (1.0 / (4.0 * kPi)) * rcp(r_out * r_out * fmax(rho_ro, rho_min));
     const double ret_b = m_return * rcp(H.mass) * rv;
     double ret = (r_out > 0.0 && rho_ro > 0.0) ? ret_a : (m_return > 0.0 ? ret_b : 0.0);
-    ret = r_out < rv ? ret : 0.0;
+    // the outer radius starts at the virial radius, an exact tie in the
+    // reference; a last-place difference between the caller's value and the
+    // recomputed virial radius is judged a tie (1e-12 relative)
+    const bool at_rv = r_out >= rv * (1.0 - 1.0e-12);
+    ret = at_rv ? 0.0 : ret;
     // stripping radius and timescale
     const double force = np.ram_force;
     const double bind = P[GAL_P_RAM_PRESSURE_FORM_FACTOR] * H.v_vir * H.v_vir;
     const double x_v = rv * inv_r_core, r_small = 1.0e-6 * rv, x_s = r_small * inv_r_core;
-    const double rho_rv = (hot_ok && rv <= r_out) ? rho0 * rcp(1.0 + x_v * x_v) : 0.0;
+    const double rho_rv = (hot_ok && at_rv) ? rho0 * rcp(1.0 + x_v * x_v) : 0.0;
     const double rho_small = (hot_ok && r_small <= r_out) ? rho0 * rcp(1.0 + x_s * x_s) : 0.0;
     const double r_star = fmin(r_core * sqrt(fmax(bind * rho0 * rcp(force) - 1.0, 0.0)), r_out);
     const double r_rp = bind * rho_rv - force >= 0.0 ? rv : (bind * rho_small - force <= 0.0 ? 0.0 : r_star);

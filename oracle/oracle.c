@@ -686,6 +686,17 @@ static double hot_density(const orc_hot *hh, double r)
   return hh->valid ? beta_density(hh, r) : 0.0;
 }
 
+/* The outer radius starts at the virial radius (hot_halo/standard.F90 sets
+   outerRadius = radiusVirial), so whether it lies inside the virial radius is
+   an exact tie in the reference, where both come from one function.  Here the
+   initial value is written by the caller and the virial radius recomputed in
+   the RHS; a last-place difference between the two must not switch the
+   returning-outflow term on, so the tie is judged to 1e-12 relative. */
+static int reaches_virial(double r_outer, double r_vir)
+{
+  return r_outer >= r_vir * (1.0 - 1.0e-12);
+}
+
 /* hotHaloRamPressureStrippingFont2008%radiusStripped
    (hot_halo/ram_pressure_stripping/Font2008.F90:115-267): the radius where
    the gravitational binding force per area, formFactor G M(<r) rho_hot(r) / r,
@@ -700,7 +711,9 @@ static double ram_pressure_radius(const double *P, const orc_halo *h, const orc_
 {
   const double rv = h->r_vir, ff = P[GAL_P_RAM_PRESSURE_FORM_FACTOR];
   const double bind = ff * h->v_vir * h->v_vir;
-  if (bind * hot_density(hh, rv) - force >= 0.0) return rv;
+  const double x_v = rv / hh->r_core;
+  const double rho_rv = (hh->valid && reaches_virial(hh->r_outer, rv)) ? hh->rho0 / (1.0 + x_v * x_v) : 0.0;
+  if (bind * rho_rv - force >= 0.0) return rv;
   if (bind * hot_density(hh, 1.0e-6 * rv) - force <= 0.0) return 0.0;
   double r = hh->r_core * sqrt(bind * hh->rho0 / force - 1.0);
   if (r > hh->r_outer) r = hh->r_outer;
@@ -726,7 +739,7 @@ static void cgm_outer_radius(const orc_sys *s, const orc_halo *h, const orc_hot 
   const double ro = y[GAL_V_HOT_OUTER_RADIUS], rv = h->r_vir;
   /* haloDynamicalTimeRate (hot_halo/outflow_reincorporation/halo_dynamical_time.F90:113-128) */
   const double m_return = y[GAL_V_HOT_OUTFLOWED_MASS] * P[GAL_P_REINCORPORATION_MULTIPLIER] / h->t_dyn;
-  if (ro < rv) {
+  if (!reaches_virial(ro, rv)) {
     double rho = hot_density(hh, ro);
     if (ro > 0.0 && rho > 0.0) {
       double fb = P[GAL_P_OMEGA_BARYON] / P[GAL_P_OMEGA_MATTER];

@@ -7,14 +7,20 @@ FP64 Cash-Karp at rel 1e-3, abs 1e-6):
     (circumgalactic_medium/accretion.F90 with accretionHalo simple) and the
     only sink is the energy radiated and carried off by the black-hole jet,
     a fraction eff/(1 - eff) <= 0.75 of the black hole's growth (ISCO
-    efficiency at most 0.42 for spin <= 0.998, black_holes/accretion.F90).
-    A Runge-Kutta step preserves linear invariants of the RHS up to rounding,
-    so the residual is held to 1e-7 of the baryon mass (1e-6 for metals, whose
-    flows through the yield can exceed the metal mass itself);
+    efficiency at most 0.42 for spin <= 0.998, black_holes/accretion.F90);
+    gas expelled by excess heating leaves the node, so that outflow is off
+    here, and so is the post-step zeroing of negative reservoirs
+    (postStepZeroNegativeMasses of the disk and spheroid components), which
+    adds the mass it clips: at rel 1e-3 a spheroid whose gas is consumed on a
+    short timescale overshoots below zero on every one of thousands of
+    attempts, and the clipping alone adds 0.4% to one node in this batch.
+    Without it a Runge-Kutta step preserves linear invariants of the RHS up to
+    rounding, so the residual is held to 1e-7 of the baryon mass (and of the
+    metal mass, whose flows through the yield can exceed the metal mass
+    itself);
   * metals are conserved: the only source is the yield of newly formed stars
     (yield x Delta(stars formed)), the only sink the gas the black hole eats
     (at most its growth / (1 - eff) of gas, whose metal fraction is below 1);
-  * nodes whose black hole does not grow conserve both to rounding.
 A dropped or doubled flux term of any operator breaks one of these at the
 level of that flux, far above 1e-7.
 """
@@ -27,10 +33,10 @@ from galacticus_amd import workloads as wl
 pytestmark = pytest.mark.gpu
 
 V = wl.V
-BARYONS = ["HOT_MASS", "HOT_OUTFLOWED_MASS", "HOT_UNACCRETED_MASS", "DISK_MASS_GAS", "DISK_MASS_STELLAR",
-           "SPHEROID_MASS_GAS", "SPHEROID_MASS_STELLAR", "BH_MASS"]
-METALS = ["HOT_ABUNDANCES", "HOT_OUTFLOWED_ABUNDANCES", "HOT_UNACCRETED_ABUNDANCES", "DISK_ABUNDANCES_GAS",
-          "DISK_ABUNDANCES_STELLAR", "SPHEROID_ABUNDANCES_GAS", "SPHEROID_ABUNDANCES_STELLAR"]
+BARYONS = ["HOT_MASS", "HOT_OUTFLOWED_MASS", "HOT_UNACCRETED_MASS", "HOT_STRIPPED_MASS", "DISK_MASS_GAS",
+           "DISK_MASS_STELLAR", "SPHEROID_MASS_GAS", "SPHEROID_MASS_STELLAR", "BH_MASS"]
+METALS = ["HOT_ABUNDANCES", "HOT_OUTFLOWED_ABUNDANCES", "HOT_UNACCRETED_ABUNDANCES", "HOT_STRIPPED_ABUNDANCES",
+          "DISK_ABUNDANCES_GAS", "DISK_ABUNDANCES_STELLAR", "SPHEROID_ABUNDANCES_GAS", "SPHEROID_ABUNDANCES_STELLAR"]
 
 
 @pytest.fixture(scope="module")
@@ -43,7 +49,12 @@ def torch_cuda():
 def test_full_batch_conserves_baryons_and_metals(torch_cuda, cooling_table):
     torch = torch_cuda
     from galacticus_amd.evolver import NodeEvolverBatched
-    p = gp.physics_params()
+    # gas expelled by excess heating leaves the node (cooling_heating.F90:216-240):
+    # the budget closes with that outflow off, as in the reference's own
+    # mass-conservation model (test-mass-conservation-standard.xml has no
+    # circumgalactic heating); the clipping of negative reservoirs is off so
+    # that the budget is an exact linear invariant of every step
+    p = gp.physics_params(**{"CGMCoolingHeating.excessHeatDrivesOutflow": 0.0, "postStepZeroNegativeMasses": 0.0})
     b = wl.isolated_halos(100000, seed=0, params=p)
     print("full batch", b.n, "nodes", flush=True)
     ev = NodeEvolverBatched(b.n, p, cooling_table, odeToleranceAbsolute=1e-6, odeToleranceRelative=1e-3)
@@ -70,9 +81,7 @@ def test_full_batch_conserves_baryons_and_metals(torch_cuda, cooling_table):
     zres = (z1 - z0) - yld * formed
     zscale = z0 + yld * np.abs(formed)
     print("metal residual / scale: min", (zres / zscale).min(), "max", (zres / zscale).max(), flush=True)
-    assert (zres <= 1e-6 * zscale).all()
-    assert (-zres <= 2.5 * np.maximum(dbh, 0.0) + 1e-6 * zscale).all()
-    quiet = dbh <= 0.0
-    print("nodes without black-hole growth", int(quiet.sum()), flush=True)
-    assert (np.abs(loss[quiet]) <= 1e-7 * scale[quiet]).all()
-    assert (np.abs(zres[quiet]) <= 1e-6 * zscale[quiet]).all()
+    assert (zres <= 1e-7 * zscale).all()
+    assert (-zres <= 2.5 * np.maximum(dbh, 0.0) + 1e-7 * zscale).all()
+    # reservoirs did go negative (the switch above matters in this batch)
+    assert (y1 < 0.0).any()

@@ -6,6 +6,8 @@ node-evolver's failure branches, against the CPU oracle.
   operators (satelliteMassLoss, barInstability, blackHolesWinds,
   CGMOuterRadiusRamPressureStripping) and with excess-heat expulsion on and
   off: |f_gpu - f_oracle| <= 1e-9 (|f_oracle| + scale).
+* Empty and subnormal reservoirs (the states enforceNonNegativity produces):
+  finite derivatives that agree with the oracle.
 * Evolution of satellites stripped by ram pressure: every node within 10x the
   ODE tolerance of the oracle, identical statuses.
 * The trial loop of node_evolver/standard.F90:587-718:
@@ -63,6 +65,45 @@ def test_rhs_operators_match_oracle(torch_cuda, cooling_table, workload, operato
     sc = oracle.scales(b.t_start, b.y, b.node_par, p)
     bad = np.abs(fg - fo) > 1e-9 * (np.abs(fo) + sc)
     assert not bad.any(), [(wl.VAR_NAMES[v], int(np.sum(bad[v]))) for v in range(30) if bad[v].any()]
+
+
+RESERVOIRS = {"hot": ["HOT_MASS", "HOT_ABUNDANCES", "HOT_ANGULAR_MOMENTUM"],
+              "disk gas": ["DISK_MASS_GAS", "DISK_ABUNDANCES_GAS"],
+              "disk": ["DISK_MASS_GAS", "DISK_ABUNDANCES_GAS", "DISK_MASS_STELLAR", "DISK_ABUNDANCES_STELLAR",
+                       "DISK_ANGULAR_MOMENTUM"],
+              "spheroid gas": ["SPHEROID_MASS_GAS", "SPHEROID_ABUNDANCES_GAS"],
+              "spheroid": ["SPHEROID_MASS_GAS", "SPHEROID_ABUNDANCES_GAS", "SPHEROID_MASS_STELLAR",
+                           "SPHEROID_ABUNDANCES_STELLAR", "SPHEROID_ANGULAR_MOMENTUM"],
+              "outflowed": ["HOT_OUTFLOWED_MASS", "HOT_OUTFLOWED_ABUNDANCES", "HOT_OUTFLOWED_ANGULAR_MOMENTUM"]}
+
+
+@pytest.mark.parametrize("workload", ["isolated", "satellites"])
+@pytest.mark.parametrize("factor", [0.0, 1e-320, 1e-300, 1e-120])
+def test_rhs_with_empty_and_subnormal_reservoirs(torch_cuda, cooling_table, workload, factor):
+    """A reservoir (its mass with its metals and angular momentum) emptied or
+    scaled down to subnormal values, as one that enforceNonNegativity lets
+    decay by halved steps becomes: every derivative finite and within
+    1e-9 (|f_oracle| + scale) of the oracle wherever the oracle's is finite
+    (the oracle's radiated-energy rate is NaN at a subnormal hot mass)."""
+    torch = torch_cuda
+    n = 1024
+    b = wl.isolated_halos(n, seed=13) if workload == "isolated" else wl.satellite_halos(n, seed=13)
+    p = gp.physics_params()
+    t = b.t_start + 0.3 * (b.t_end - b.t_start)
+    sc = oracle.scales(b.t_start, b.y, b.node_par, p)
+    ev = _evolver(n, p, cooling_table)
+    dev = lambda a: torch.from_numpy(np.ascontiguousarray(a)).cuda()
+    for name, variables in RESERVOIRS.items():
+        print(f"rhs {workload}: {name} reservoir x {factor}", flush=True)
+        y = b.y.copy()
+        for v in variables:
+            y[V[v]] *= factor
+        fg = ev.rhs_device(dev(t), dev(y), dev(b.node_par)).cpu().numpy()
+        fo = oracle.rhs(t, y, b.node_par, p, cooling_table, root_tol_rel=1e-15)
+        assert np.isfinite(fg).all(), (name, [wl.VAR_NAMES[v] for v in range(30) if not np.isfinite(fg[v]).all()])
+        ok = np.isfinite(fo)
+        bad = ok & (np.abs(fg - fo) > 1e-9 * (np.abs(fo) + sc))
+        assert not bad.any(), (name, [wl.VAR_NAMES[v] for v in range(30) if bad[v].any()])
 
 
 def test_satellite_stripping_evolution_matches_oracle(torch_cuda, cooling_table):

@@ -186,6 +186,9 @@ def test_evolve_matches_oracle(torch_cuda, cooling_table, reuse):
     bad, med, mx = _compare_evolution(yg, yo, sco, 1e-6, 1e-3)
     same_steps = np.mean(cntg[0] == cnto[0])
     print(f"median rel {med:.3e} max rel {mx:.3e} same-step-count fraction {same_steps:.4f}", flush=True)
+    for v, i in zip(*np.nonzero(bad)):
+        print(f"  beyond 10x tolerance: node {i} {wl.VAR_NAMES[v]} gpu {yg[v, i]:.9e} oracle {yo[v, i]:.9e} "
+              f"attempts {cntg[0][i]}/{cnto[0][i]}", flush=True)
     assert not bad.any(), int(bad.sum())
     assert med < 1e-9
     assert same_steps > 0.95
