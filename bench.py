@@ -54,33 +54,62 @@ def log(msg):
     print(msg, file=sys.stderr, flush=True)
 
 
+def _cpu_model() -> str:
+    try:
+        with open("/proc/cpuinfo", encoding="utf-8") as f:
+            for line in f:
+                if line.startswith("model name"):
+                    return line.split(":", 1)[1].strip()
+    except OSError:
+        pass
+    return "unknown"
+
+
 def cpu_baseline(batch, physics, table, tol_abs, tol_rel, flags, target_seconds=15.0):
-    """The CPU restatement (oracle) on a bounded sample of the same batch, all host cores."""
+    """The CPU restatement (oracle) on a bounded sample of the same batch: one
+    warm-up run and the median of five timed runs on all host cores, and the
+    same on one core, built for this host (-O3 -march=native)."""
+    import tempfile
     import oracle
+    build = "prebuilt oracle/lib/liboracle.so (-O3 -march=x86-64-v3)"
+    try:
+        oracle.use_library(oracle.build_native(os.path.join(tempfile.gettempdir(), "galacticus_amd_oracle")))
+        build = "oracle/oracle.c built on this host with gcc -O3 -march=native -fopenmp -fno-math-errno"
+    except Exception as e:      # a host without gcc keeps the portable build
+        log(f"cpu baseline: native build failed ({e}); using the prebuilt oracle")
     cores = len(os.sched_getaffinity(0)) if hasattr(os, "sched_getaffinity") else os.cpu_count()
     omp = os.environ.get("OMP_NUM_THREADS")
     if omp and omp.isdigit():
         cores = min(cores, int(omp))
     cores = max(1, cores)
+
+    def run(m, threads):
+        sl = slice(0, m)
+        t = time.perf_counter()
+        _, _, _, st, _ = oracle.evolve(batch.y[:, sl], batch.node_par[:, sl], batch.t_start[sl], batch.t_end[sl],
+                                       batch.h_step[sl], physics, table, tol_abs, tol_rel, flags, root_tol_rel=1e-6,
+                                       nthreads=threads)
+        return time.perf_counter() - t, float(np.sum(np.abs(batch.t_end[sl] - batch.t_start[sl])))
+
     # calibrate on 200 nodes with one thread
     m0 = min(200, batch.n)
-    sl = slice(0, m0)
-    t = time.perf_counter()
-    oracle.evolve(batch.y[:, sl], batch.node_par[:, sl], batch.t_start[sl], batch.t_end[sl], batch.h_step[sl],
-                  physics, table, tol_abs, tol_rel, flags, root_tol_rel=1e-6, nthreads=1)
-    per_node = (time.perf_counter() - t) / m0
-    m = int(min(batch.n, max(m0, target_seconds * cores / max(per_node, 1e-9))))
-    sl = slice(0, m)
-    t = time.perf_counter()
-    _, _, _, st, _ = oracle.evolve(batch.y[:, sl], batch.node_par[:, sl], batch.t_start[sl], batch.t_end[sl],
-                                   batch.h_step[sl], physics, table, tol_abs, tol_rel, flags, root_tol_rel=1e-6,
-                                   nthreads=cores)
-    wall = time.perf_counter() - t
-    gyr = float(np.sum(np.abs(batch.t_end[sl] - batch.t_start[sl])))
-    return {"value": gyr / wall, "unit": "node-Gyr/s", "cores": cores, "kind": "port",
-            "sample": f"first {m} of the {batch.n} nodes ({gyr:.1f} node-Gyr), oracle/oracle.c with OpenMP "
-                      f"schedule(dynamic) over nodes, cooling-radius root tolerance 1e-6 as the reference, "
-                      f"{wall:.2f} s wall"}
+    per_node = run(m0, 1)[0] / m0
+    out = {}
+    for label, threads, share in (("all", cores, 0.65), ("one", 1, 0.35)):
+        per_run = target_seconds * share / 6.0
+        m = int(min(batch.n, max(m0, per_run * threads / max(per_node, 1e-9))))
+        walls = [run(m, threads) for _ in range(6)]
+        gyr = walls[0][1]
+        med = float(np.median([w for w, _ in walls[1:]]))
+        out[label] = (gyr / med, m, gyr, med)
+    v_all, m_all, gyr_all, med_all = out["all"]
+    v_one, m_one, gyr_one, med_one = out["one"]
+    return {"value": v_all, "unit": "node-Gyr/s", "cores": cores, "kind": "port",
+            "one_core_value": v_one, "nproc": os.cpu_count(), "cpu_model": _cpu_model(),
+            "sample": f"first {m_all} of the {batch.n} nodes ({gyr_all:.1f} node-Gyr) on {cores} threads, median "
+                      f"{med_all:.2f} s of 5 runs after one warm-up; first {m_one} nodes ({gyr_one:.1f} node-Gyr) on "
+                      f"one core, median {med_one:.2f} s; {build}; OpenMP schedule(dynamic) over nodes; "
+                      f"cooling-radius root tolerance 1e-6 as the reference"}
 
 
 MASS_HEADS = ("DISK_MASS_STELLAR", "SPHEROID_MASS_STELLAR", "DISK_MASS_GAS", "SPHEROID_MASS_GAS", "HOT_MASS", "BH_MASS")
@@ -685,6 +714,21 @@ def main():
         launch_step(i, i)
     torch.cuda.synchronize()
     log(f"warmup done ({args.warmup} steps)")
+    # One batch alone on one stream: the rate without any overlap of batches.
+    single_ms = []
+    for rep in range(3):
+        ys1, hs1 = y0.clone(), h0.clone()
+        e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+        torch.cuda.synchronize()
+        e0.record(streams[0])
+        evs[0].evolve_device(ys1, npar, t0, t1, hs1, scale=scales[0], status=statuses[0], counters=counters[0],
+                             stream=streams[0])
+        e1.record(streams[0])
+        torch.cuda.synchronize()
+        single_ms.append(e0.elapsed_time(e1))
+        del ys1, hs1
+    single_launch_ms = float(np.median(single_ms))
+    log(f"one batch alone: {single_launch_ms:.3f} ms ({node_gyr / (single_launch_ms * 1e-3):.4g} node-Gyr/s)")
 
     starts = [torch.cuda.Event(enable_timing=True) for _ in range(args.steps)]
     ends = [torch.cuda.Event(enable_timing=True) for _ in range(args.steps)]
@@ -707,10 +751,13 @@ def main():
             sys.exit(1)
     cnt = counters[0].view(3, n).cpu().numpy()
     attempts = int(cnt[0].sum())
-    flop_step = gflops.attempt_flops()
+    rhs_evals = int(cnt[2].sum())
+    # algorithmic FLOPs of a launch from its measured counters: F_rhs per RHS
+    # evaluation (rejected attempts take five, accepted six) and F_comb per attempt
+    flop_launch = gflops.evaluated_flops(rhs_evals, attempts)
     state_bytes = 8 * (gp.NVAR + npar.shape[0] + 3) + 8 * (2 * gp.NVAR + 1) + 4 + 3 * 8
-    achieved_tflops = attempts * flop_step / (kernel_ms * 1e-3) / 1e12
-    aggregate_tflops = attempts * args.steps * flop_step / wall / 1e12
+    achieved_tflops = flop_launch / (kernel_ms * 1e-3) / 1e12
+    aggregate_tflops = flop_launch * args.steps / wall / 1e12
     log(f"step {ms_per_step:.3f} ms (one launch {kernel_ms:.3f} ms, {n_streams} streams), attempts {attempts}, "
         f"max/node {cnt[0].max()}")
 
@@ -735,16 +782,25 @@ def main():
             "launch": {"nodes_per_workgroup": launch[0], "workgroups_per_cu": launch[1], "streams": n_streams,
                        "tuning_ms_per_step": tuning},
         },
+        "single_launch": {
+            "ms": single_launch_ms,
+            "value": node_gyr / (single_launch_ms * 1e-3),
+            "unit": "node-Gyr/s",
+            "note": "one batch alone on one stream (median of 3); value overlaps independent batches on "
+                    f"{n_streams} streams",
+        },
         "roofline": {
-            "bound": "mfma",
-            "compute": "fp64 (VALU; the FP64 vector and matrix peaks are both 78.6 TFLOP/s on MI355X)",
+            "bound": "fp64-valu",
+            "compute": "FP64 vector ALU (the MI355X FP64 vector peak, 78.6 TFLOP/s)",
             "achieved": achieved_tflops,
             "peak": FP64_PEAK_TFLOPS,
             "unit": "TFLOP/s",
             "frac": achieved_tflops / FP64_PEAK_TFLOPS,
             "traffic": HBM_TRAFFIC_1E5.get(tuple(launch)) if n == 100000 and args.seed == 0 else None,
             "traffic_unit": "bytes per launch (rocprofv3 counters, calibrated)",
-            "flop_per_attempt": flop_step,
+            "flop_per_rhs_evaluation": gflops.rhs_flops(),
+            "flop_per_attempt_combination": gflops.combination_flops(),
+            "flop_per_launch": flop_launch,
             "attempts_per_launch": attempts,
             "accepted_steps_per_launch": int(cnt[1].sum()),
             "rhs_evaluations_per_launch": int(cnt[2].sum()),

@@ -85,5 +85,13 @@ def attempt_flops() -> int:
     return 6 * rhs_flops() + combination_flops()
 
 
+def evaluated_flops(rhs_evaluations: int, attempts: int) -> int:
+    """Algorithmic FLOPs of a launch from its measured counters: every RHS
+    evaluation (six for an accepted attempt, five for a rejected one, whose
+    end-point derivative is never taken, plus the first of each node) and the
+    stage combinations and error norm of every attempt."""
+    return int(rhs_evaluations) * rhs_flops() + int(attempts) * combination_flops()
+
+
 if __name__ == "__main__":
     print("F_rhs", rhs_flops(), "F_comb", combination_flops(), "FLOP_step", attempt_flops())

@@ -42,6 +42,27 @@ def build() -> str:
     return _LIB_PATH
 
 
+def build_native(dest_dir: str) -> str:
+    """Compile the oracle for the host it runs on (-march=native, the
+    reference's -O3 and -fno-math-errno, OpenMP) into dest_dir; returns the
+    path.  Used for the CPU baseline, where the prebuilt library (portable
+    x86-64-v3) would understate a newer host."""
+    os.makedirs(dest_dir, exist_ok=True)
+    out = os.path.join(dest_dir, "liboracle_native.so")
+    cmd = ["gcc", "-O3", "-march=native", "-fPIC", "-fopenmp", "-ffp-contract=off", "-fno-math-errno", "-std=c11",
+           "-shared", "-o", out, os.path.join(_HERE, "oracle.c"), "-lm"]
+    subprocess.run(cmd, check=True, capture_output=True)
+    return out
+
+
+def use_library(path: str) -> None:
+    """Load the oracle from another build of oracle.c (same ABI)."""
+    global _LIB_PATH, _lib
+    _LIB_PATH = path
+    _lib = None
+    lib()
+
+
 def lib():
     global _lib
     if _lib is None:
