@@ -40,6 +40,7 @@
 #include <cstring>
 #include <cmath>
 #include <algorithm>
+#include <mutex>
 
 #include "../../include/gal_ode.h"
 #include "constants.hpp"
@@ -495,9 +496,7 @@ static int half_power_code(double alpha) {
   return kNoHalfPower;
 }
 
-static void derive_constants(gal_ode_batch* h) {
-  const double* p = h->params;
-  DerivedConstants& D = h->D;
+static void derive_constants(const double* p, DerivedConstants& D) {
   const double om = p[GAL_P_OMEGA_MATTER], ol = p[GAL_P_OMEGA_DARK_ENERGY], h0 = p[GAL_P_HUBBLE_CONSTANT];
   D.om = om;
   D.ol = ol;
@@ -660,30 +659,36 @@ int gal_ode_batch_destroy(gal_ode_batch* h) {
   return GAL_OK;
 }
 
-int gal_ode_batch_set_physics(gal_ode_batch* h, const double* params, int n_params) {
-  if (!h || !params || n_params != GAL_NPARAM) return GAL_E_INVALID;
+static bool physics_valid(const double* params) {
   const int mode = (int)params[GAL_P_PHYSICS_MODE];
-  if (mode < GAL_PHYSICS_STANDARD || mode > GAL_PHYSICS_TEST) return GAL_E_INVALID;
+  if (mode < GAL_PHYSICS_STANDARD || mode > GAL_PHYSICS_TEST) return false;
   if (!(params[GAL_P_OMEGA_MATTER] > 0.0) || !(params[GAL_P_OMEGA_DARK_ENERGY] > 0.0) ||
       !(params[GAL_P_HUBBLE_CONSTANT] > 0.0) || !(params[GAL_P_FB_DISK_VELOCITY] > 0.0) ||
       !(params[GAL_P_FB_SPHEROID_VELOCITY] > 0.0) || !(params[GAL_P_FB_DISK_TIMESCALE_MINIMUM] > 0.0) ||
       !(params[GAL_P_FB_SPHEROID_TIMESCALE_MINIMUM] > 0.0) || !(params[GAL_P_CORE_RADIUS_OVER_VIRIAL] > 0.0) ||
       !(params[GAL_P_FRACTION_LOSS_ANGULAR_MOMENTUM] < 1.0) || !(params[GAL_P_OPERATORS] >= 0.0) ||
       !(params[GAL_P_OPERATORS] <= 15.0))
-    return GAL_E_INVALID;
+    return false;
+  return true;
+}
+
+int gal_ode_batch_set_physics(gal_ode_batch* h, const double* params, int n_params) {
+  if (!h || !params || n_params != GAL_NPARAM) return GAL_E_INVALID;
+  if (!physics_valid(params)) return GAL_E_INVALID;
   std::memcpy(h->params, params, sizeof(double) * GAL_NPARAM);
-  derive_constants(h);
+  derive_constants(h->params, h->D);
   HIP_TRY(hipMemcpy(h->d_params, h->params, sizeof(double) * GAL_NPARAM, hipMemcpyHostToDevice));
   h->have_physics = true;
   return GAL_OK;
 }
 
-int gal_ode_batch_set_cooling_table(gal_ode_batch* h, const double* ln_lambda, int n_t, double ln_t_min,
-                                    double d_ln_t, int n_z, double ln_z_min, double d_ln_z) {
-  if (!h || !ln_lambda || n_t < 2 || n_z < 2 || n_t > GAL_COOLING_NT_MAX || n_z > GAL_COOLING_NZ_MAX ||
-      !(d_ln_t > 0.0) || !(d_ln_z > 0.0))
-    return GAL_E_INVALID;
-  CoolingDesc& c = h->cd;
+static bool cooling_table_valid(const double* ln_lambda, int n_t, double d_ln_t, int n_z, double d_ln_z) {
+  return ln_lambda && n_t >= 2 && n_z >= 2 && n_t <= GAL_COOLING_NT_MAX && n_z <= GAL_COOLING_NZ_MAX &&
+         d_ln_t > 0.0 && d_ln_z > 0.0;
+}
+
+static void cooling_desc(CoolingDesc& c, int n_t, double ln_t_min, double d_ln_t, int n_z, double ln_z_min,
+                         double d_ln_z) {
   c.n_t = n_t;
   c.n_z = n_z;
   c.ln_t_min = ln_t_min;
@@ -696,6 +701,12 @@ int gal_ode_batch_set_cooling_table(gal_ode_batch* h, const double* ln_lambda, i
   c.t_max = std::exp(ln_t_min + d_ln_t * (n_t - 1));
   c.z_min = std::exp(ln_z_min);
   c.z_max = std::exp(ln_z_min + d_ln_z * (n_z - 1));
+}
+
+int gal_ode_batch_set_cooling_table(gal_ode_batch* h, const double* ln_lambda, int n_t, double ln_t_min,
+                                    double d_ln_t, int n_z, double ln_z_min, double d_ln_z) {
+  if (!h || !cooling_table_valid(ln_lambda, n_t, d_ln_t, n_z, d_ln_z)) return GAL_E_INVALID;
+  cooling_desc(h->cd, n_t, ln_t_min, d_ln_t, n_z, ln_z_min, d_ln_z);
   HIP_TRY(hipMemcpy(h->d_table, ln_lambda, sizeof(double) * n_t * n_z, hipMemcpyHostToDevice));
   h->have_table = true;
   return GAL_OK;
@@ -898,6 +909,119 @@ int gal_ode_batch_autotune(gal_ode_batch* h, int n, int ld, double* y, const dou
 }
 
 }  // extern "C"
+
+// ============================================================================
+// Allocation-free evolve
+// ============================================================================
+// The handle path keeps physics parameters, the cooling table and the claim
+// counter in device buffers that gal_ode_batch_create allocates.  Callers
+// that cannot allocate (code under graph capture, serving loops, harnesses
+// that forbid hipMalloc) use gal_ode_evolve_static_d instead: its scratch is
+// this file-scope array, one copy per GPU, held for the life of the process:
+//   [0, 8)                         claim counter
+//   [8, 8 + 8*GAL_NPARAM)          physics parameters
+//   then                           cooling table, up to NT_MAX x NZ_MAX doubles
+// Every call on a GPU reuses that GPU's copy; calls that overlap on one GPU
+// race on it.
+namespace {
+constexpr size_t kStaticTableDoubles = (size_t)GAL_COOLING_NT_MAX * GAL_COOLING_NZ_MAX;
+constexpr size_t kStaticScratchDoubles = 1 + GAL_NPARAM + kStaticTableDoubles;
+constexpr int kMaxDevices = 64;
+std::mutex g_static_mutex;
+void* g_static_scratch[kMaxDevices] = {};
+int g_static_cus[kMaxDevices] = {};
+}  // namespace
+
+static __device__ __align__(256) double scratch[kStaticScratchDoubles];
+
+// Once per device, with that device current: the device address of this
+// device's copy of the scratch array (the array's name is a host-side
+// stand-in, not a device address), the CU count and the LDS attribute of the
+// kernel the entry point launches.
+static int static_scratch(int device, void** ptr, int* n_cus) {
+  if (device < 0 || device >= kMaxDevices) return GAL_E_INVALID;
+  std::lock_guard<std::mutex> lock(g_static_mutex);
+  if (!g_static_scratch[device]) {
+    void* p = nullptr;
+    if (hipGetSymbolAddress(&p, HIP_SYMBOL(scratch)) != hipSuccess || !p) {
+      (void)hipGetLastError();
+      return GAL_E_HIP;
+    }
+    int cus = 0;
+    if (hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, device) != hipSuccess || cus < 1)
+      return GAL_E_HIP;
+    if (hipFuncSetAttribute((const void*)evolve_kernel<64, true, true>, hipFuncAttributeMaxDynamicSharedMemorySize,
+                            (int)lds_bytes<64, true, true>()) != hipSuccess) {
+      (void)hipGetLastError();
+      return GAL_E_HIP;
+    }
+    g_static_cus[device] = cus;
+    g_static_scratch[device] = p;
+  }
+  *ptr = g_static_scratch[device];
+  *n_cus = g_static_cus[device];
+  return GAL_OK;
+}
+
+extern "C" int gal_ode_evolve_static_d(const double* params, int n_params, const double* ln_lambda, int n_t,
+                                       double ln_t_min, double d_ln_t, int n_z, double ln_z_min, double d_ln_z,
+                                       double tol_abs, double tol_rel, int flags, int n, int ld, double* y,
+                                       const double* node_par, const double* t_start, const double* t_end,
+                                       double* h_step, double* scale, int* status, int64_t* counters, void* stream) {
+  if (!params || n_params != GAL_NPARAM || !physics_valid(params)) return GAL_E_INVALID;
+  if (!cooling_table_valid(ln_lambda, n_t, d_ln_t, n_z, d_ln_z)) {
+    if (ln_lambda && ((size_t)n_t * (size_t)n_z > kStaticTableDoubles))
+      fprintf(stderr, "gal_ode_evolve_static_d: cooling table %d x %d exceeds the %d x %d of the static scratch\n",
+              n_t, n_z, GAL_COOLING_NT_MAX, GAL_COOLING_NZ_MAX);
+    return GAL_E_INVALID;
+  }
+  if (tol_abs < 0.0 || tol_rel < 0.0 || (tol_abs <= 0.0 && tol_rel <= 0.0)) return GAL_E_INVALID;
+  if (n < 0 || ld < n) return GAL_E_INVALID;
+  if (n == 0) return GAL_OK;
+  if (!y || !node_par || !t_start || !t_end || !h_step || !status) return GAL_E_INVALID;
+  if ((flags & GAL_FLAG_SCALES_PROVIDED) && !scale) return GAL_E_INVALID;
+  int device = 0;
+  HIP_TRY(hipGetDevice(&device));
+  void* base = nullptr;
+  int n_cus = 0;
+  int rc = static_scratch(device, &base, &n_cus);
+  if (rc != GAL_OK) return rc;
+  hipStream_t s = (hipStream_t)stream;
+  unsigned int* work = reinterpret_cast<unsigned int*>(base);
+  double* d_params = reinterpret_cast<double*>(base) + 1;
+  double* d_table = d_params + GAL_NPARAM;
+  HIP_TRY(hipMemcpyAsync(d_params, params, sizeof(double) * GAL_NPARAM, hipMemcpyHostToDevice, s));
+  HIP_TRY(hipMemcpyAsync(d_table, ln_lambda, sizeof(double) * (size_t)n_t * n_z, hipMemcpyHostToDevice, s));
+  EvolveArgs a{};
+  a.n = n;
+  a.n_claim = n;
+  a.ld = ld;
+  a.y = y;
+  a.np = node_par;
+  a.t_start = t_start;
+  a.t_end = t_end;
+  a.h_step = h_step;
+  a.scale = scale;
+  a.status = status;
+  a.counters = reinterpret_cast<long long*>(counters);
+  a.order = nullptr;             // claims in node order
+  a.n_claim_dev = nullptr;
+  a.work = work;
+  a.P = d_params;
+  a.tab = d_table;
+  derive_constants(params, a.D);
+  cooling_desc(a.cd, n_t, ln_t_min, d_ln_t, n_z, ln_z_min, d_ln_z);
+  a.eps_abs = tol_abs;
+  a.eps_rel = tol_rel;
+  a.flags = flags;
+  a.mode = (int)params[GAL_P_PHYSICS_MODE];
+  // every lane vector in LDS, one workgroup per CU: no workspace beyond the scratch
+  constexpr size_t lds_all = lds_bytes<64, true, true>();
+  hipLaunchKernelGGL(reset_counter_kernel, dim3(1), dim3(64), 0, s, work);
+  hipLaunchKernelGGL((evolve_kernel<64, true, true>), dim3(n_cus), dim3(64), lds_all, s, a);
+  HIP_TRY(hipGetLastError());
+  return GAL_OK;
+}
 
 int gal_ode_evolve_subset(gal_ode_batch* h, int n, int ld, double* y, const double* node_par, const double* t_start,
                           const double* t_end, double* h_step, double* scale, int* status, int64_t* counters,
