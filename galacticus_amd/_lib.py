@@ -20,7 +20,7 @@ EXPORTS = [
     "gal_ode_version", "gal_ode_device_check", "gal_ode_batch_create", "gal_ode_batch_destroy",
     "gal_ode_batch_set_physics", "gal_ode_batch_set_cooling_table", "gal_ode_batch_set_launch",
     "gal_ode_batch_evolve_d", "gal_ode_batch_evolve_h", "gal_ode_rhs_d", "gal_ode_hadjust_d",
-    "gal_ode_batch_autotune", "gal_ode_evolve_static_d",
+    "gal_ode_batch_autotune", "gal_ode_evolve_static_d", "gal_ode_batch_set_rhs_waves", "gal_ode_batch_get_launch",
 ]
 
 _lib = None
@@ -63,6 +63,8 @@ def load():
     L.gal_ode_hadjust_d.argtypes = [c_int, c_int, c_int, c_double, c_double, c_double, c_double, vp, vp, vp, vp,
                                     vp, vp, vp, vp]
     L.gal_ode_batch_autotune.argtypes = [vp, c_int, c_int, vp, vp, vp, vp, vp, vp, vp, vp, ip, ip]
+    L.gal_ode_batch_set_rhs_waves.argtypes = [vp, c_int]
+    L.gal_ode_batch_get_launch.argtypes = [vp, ip, ip, ip]
     L.gal_ode_evolve_static_d.argtypes = [dp, c_int, dp, c_int, c_double, c_double, c_int, c_double, c_double,
                                           c_double, c_double, c_int, c_int, c_int, vp, vp, vp, vp, vp, vp, vp, vp,
                                           vp]

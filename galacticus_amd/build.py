@@ -20,19 +20,44 @@ HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
 # -packed-fp32-ops off: hipcc must not emit packed FP32 instructions.
 HIP_FLAGS = ["--offload-arch=gfx950", "-O3", "-std=c++17", "-fPIC", "-Xclang", "-target-feature", "-Xclang",
              "-packed-fp32-ops", "-Wno-unused-result"]
+# Per-source flags.  The integrator kernel is one loop around the whole RHS;
+# machine-level loop-invariant code motion hoists the RHS's 64-bit constants
+# (polynomial coefficients of exp/log and the like) out of it and holds them
+# in registers for the whole integration, which spills live values to
+# scratch memory.  Materialising a constant where it is used costs the same
+# two moves and no register.
+SOURCE_FLAGS = {"gal_ode.hip": ["-mllvm", "-disable-machine-licm"]}
 
 
 def build(verbose: bool = False) -> str:
     os.makedirs(LIBDIR, exist_ok=True)
-    srcs = [os.path.join(CSRC, s) for s in SOURCES if os.path.exists(os.path.join(CSRC, s))]
+    objdir = os.path.join(LIBDIR, "obj")
+    os.makedirs(objdir, exist_ok=True)
+    srcs = [s for s in SOURCES if os.path.exists(os.path.join(CSRC, s))]
+    procs, objs = [], []
+    for s in srcs:
+        obj = os.path.join(objdir, s + ".o")
+        cmd = [HIPCC, *HIP_FLAGS, *SOURCE_FLAGS.get(s, []), "-c", "-o", obj, os.path.join(CSRC, s)]
+        if verbose:
+            print(" ".join(cmd), flush=True)
+        procs.append((s, subprocess.Popen(cmd, stdout=subprocess.PIPE, stderr=subprocess.STDOUT, text=True)))
+        objs.append(obj)
+    failed = []
+    for s, p in procs:
+        out, _ = p.communicate()
+        if p.returncode != 0:
+            sys.stderr.write(out)
+            failed.append(s)
+    if failed:
+        raise RuntimeError("hipcc failed compiling " + ", ".join(failed))
     tmp = LIB + ".tmp"
-    cmd = [HIPCC, *HIP_FLAGS, "-shared", "-o", tmp, *srcs]
+    cmd = [HIPCC, "--offload-arch=gfx950", "-fPIC", "-shared", "-o", tmp, *objs]
     if verbose:
         print(" ".join(cmd), flush=True)
     r = subprocess.run(cmd, capture_output=True, text=True)
     if r.returncode != 0:
         sys.stderr.write(r.stdout + r.stderr)
-        raise RuntimeError("hipcc failed building libgalacticus_amd.so")
+        raise RuntimeError("hipcc failed linking libgalacticus_amd.so")
     os.replace(tmp, LIB)
     return LIB
 

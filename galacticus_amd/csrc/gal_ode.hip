@@ -35,6 +35,7 @@
 // addressable private memory, so the long dependent chain of a node never
 // waits on scratch.
 #include <hip/hip_runtime.h>
+#include <cstddef>
 #include <cstdint>
 #include <cstdio>
 #include <cstring>
@@ -119,9 +120,87 @@ __device__ __forceinline__ void write_counters(const EvolveArgs& a, int node, lo
 //                 (30 KiB: four workgroups per CU, one wave on every SIMD).
 // The arithmetic is the same in all three; only where the operands wait
 // between phases differs.
+//
+// WAVES = 2 gives each group of BLOCK nodes a second wave, on another SIMD of
+// the CU, that computes the hot-halo half of every RHS while the first wave
+// computes the galaxy half and runs the integrator.  One wave64 instruction
+// costs the same issue time however many of its lanes are active, so a
+// node's chain of RHS evaluations -- which sets the launch time when a few
+// stiff nodes take thousands of attempts -- shortens by the share of the
+// RHS the second wave takes.  Per RHS the first wave publishes the stage
+// state in LDS and a command (compute, skip, quit); barrier; both halves;
+// barrier; the first wave adds the second's derivatives.  Only the first
+// wave touches the integrator's state; the exchange area is written by one
+// wave and read by the other on opposite sides of a barrier.
+constexpr int kXchgVectors = kNV + 1 + 7 + kNV;   // stage state and time, node parameters, hot-half derivatives
+enum : int { kCmdCompute = 1, kCmdSkip = 2, kCmdQuit = 3 };
+
 template <int BLOCK, bool YLDS, bool KLDS>
-__global__ __launch_bounds__(BLOCK, 1) void evolve_kernel(EvolveArgs a) {
+constexpr size_t lds_vector_bytes() {
+  return sizeof(double) * ((YLDS ? 2 : 0) + (KLDS ? kStages : 0)) * kNV * BLOCK;
+}
+
+// The kernel's one argument starts the kernarg segment: the derived constants
+// and table descriptor are read from there through an offset the compiler
+// cannot prove constant, so their scalar loads stay inside each RHS instead
+// of being hoisted and held (and spilled) across the whole integration.
+struct UniformRefs {
+  const double* P;
+  const DerivedConstants* D;
+  const CoolingDesc* C;
+};
+__device__ __forceinline__ UniformRefs uniform_refs(const double* P) {
+  int opaque = 0;
+  asm volatile("" : "+s"(opaque));
+  opaque = __builtin_amdgcn_readfirstlane(opaque);
+  const char* kargs = (const char*)__builtin_amdgcn_kernarg_segment_ptr() + opaque;
+  return UniformRefs{P + opaque, (const DerivedConstants*)(kargs + offsetof(EvolveArgs, D)),
+                     (const CoolingDesc*)(kargs + offsetof(EvolveArgs, cd))};
+}
+
+template <int BLOCK>
+__device__ __forceinline__ void rhs_coprocessor(const double* P, const double* __restrict__ tab, int mode,
+                                                double* __restrict__ xchg, volatile int* cmd) {
+  const int lane = threadIdx.x - BLOCK;
+  double* __restrict__ X = xchg + lane;
+  for (;;) {
+    __syncthreads();   // stage state published
+    const int c = __builtin_amdgcn_readfirstlane(*cmd);
+    if (c == kCmdQuit) break;
+    if (c == kCmdCompute) {
+      double yy[kNV], fh[kNV];
+#pragma unroll
+      for (int v = 0; v < kNV; ++v) yy[v] = X[v * BLOCK];
+      const double ts = X[kNV * BLOCK];
+      NodePar np;
+      np.t_ref = X[(kNV + 1) * BLOCK];
+      np.m_ref = X[(kNV + 2) * BLOCK];
+      np.m_rate = X[(kNV + 3) * BLOCK];
+      np.spin_rate = X[(kNV + 4) * BLOCK];
+      np.ram_force = X[(kNV + 5) * BLOCK];
+      np.v_host = X[(kNV + 6) * BLOCK];
+      np.satellite = X[(kNV + 7) * BLOCK] != 0.0;
+      const UniformRefs u = uniform_refs(P);
+      rhs_part<kRhsHalo>(u.P, *u.D, tab, *u.C, mode, np, ts, yy, fh);
+#pragma unroll
+      for (int v = 0; v < kNV; ++v) X[(kNV + 8 + v) * BLOCK] = fh[v];
+    }
+    __syncthreads();   // hot-half derivatives published
+  }
+}
+
+template <int BLOCK, bool YLDS, bool KLDS, int WAVES = 1>
+__global__ __launch_bounds__(BLOCK * WAVES, 1) void evolve_kernel(EvolveArgs a) {
   extern __shared__ double lds_k[];
+  __shared__ int xchg_cmd;
+  double* __restrict__ xchg = lds_k + lds_vector_bytes<BLOCK, YLDS, KLDS>() / sizeof(double);
+  if constexpr (WAVES == 2) {
+    if (threadIdx.x >= BLOCK) {
+      rhs_coprocessor<BLOCK>(a.P, a.tab, a.mode, xchg, &xchg_cmd);
+      return;
+    }
+  }
+  double* __restrict__ XC = xchg + threadIdx.x;    // this lane's column of the exchange area
   double* __restrict__ K = lds_k + threadIdx.x;   // K[(s*kNV + v)*BLOCK] when KLDS
   // In the HBM workspace the stage vectors are reached through a buffer
   // resource over the workgroup's slab: the lane offset sits in one VGPR and
@@ -163,7 +242,65 @@ __global__ __launch_bounds__(BLOCK, 1) void evolve_kernel(EvolveArgs a) {
   double* __restrict__ SL = YL + kNV * BLOCK;
 #define Y(v) YL[(v) * ys]
 #define SC(v) SL[(v) * ys]
-  double ytmp[kNV], kout[kNV];
+  // Partial sums over the stage derivatives already in memory, formed while
+  // an RHS is in flight: for phases 0-3 the next stage's input sum without
+  // its newest term; for phase 4 the fifth-order solution's sum without k6.
+  // Each is completed, in the order of the full sum, once the newest
+  // derivative is known.
+  // Coefficients of the newest term of the stage input that follows phase ph (0-3).
+  auto stage_b_new = [](int ph) {
+    return ph == 0 ? ck::b32 : (ph == 1 ? ck::b43 : (ph == 2 ? ck::b54 : ck::b65));
+  };
+  auto stage_a_new = [](int ph) { return ph == 0 ? ck::a3 : (ph == 1 ? ck::a4 : (ph == 2 ? ck::a5 : ck::a6)); };
+  // The LDS operands are gathered ten variables at a time before the
+  // arithmetic, so that their reads are in flight together rather than one
+  // read-and-wait per variable.
+  constexpr int kChunk = 10;
+  static_assert(kNV % kChunk == 0, "chunks must tile the state");
+  auto stage_partials = [&](double (&sp)[kNV], int ph, bool act) {
+    if (!act) return;
+#pragma unroll
+    for (int c = 0; c < kNV; c += kChunk) {
+      double g0[kChunk], g1[kChunk], g2[kChunk], g3[kChunk];
+      switch (ph) {
+        case 0:
+#pragma unroll
+          for (int i = 0; i < kChunk; ++i) g0[i] = KS(0, c + i);
+#pragma unroll
+          for (int i = 0; i < kChunk; ++i) sp[c + i] = ck::b31 * g0[i];
+          break;
+        case 1:
+#pragma unroll
+          for (int i = 0; i < kChunk; ++i) { g0[i] = KS(0, c + i); g1[i] = KS(1, c + i); }
+#pragma unroll
+          for (int i = 0; i < kChunk; ++i) sp[c + i] = ck::b41 * g0[i] + ck::b42 * g1[i];
+          break;
+        case 2:
+#pragma unroll
+          for (int i = 0; i < kChunk; ++i) { g0[i] = KS(0, c + i); g1[i] = KS(1, c + i); g2[i] = KS(2, c + i); }
+#pragma unroll
+          for (int i = 0; i < kChunk; ++i) sp[c + i] = ck::b51 * g0[i] + ck::b52 * g1[i] + ck::b53 * g2[i];
+          break;
+        case 3:
+#pragma unroll
+          for (int i = 0; i < kChunk; ++i) {
+            g0[i] = KS(0, c + i); g1[i] = KS(1, c + i); g2[i] = KS(2, c + i); g3[i] = KS(3, c + i);
+          }
+#pragma unroll
+          for (int i = 0; i < kChunk; ++i)
+            sp[c + i] = ck::b61 * g0[i] + ck::b62 * g1[i] + ck::b63 * g2[i] + ck::b64 * g3[i];
+          break;
+        case 4:
+#pragma unroll
+          for (int i = 0; i < kChunk; ++i) { g0[i] = KS(0, c + i); g2[i] = KS(2, c + i); g3[i] = KS(3, c + i); }
+#pragma unroll
+          for (int i = 0; i < kChunk; ++i) sp[c + i] = ck::c1 * g0[i] + ck::c3 * g2[i] + ck::c4 * g3[i];
+          break;
+        default:
+          break;
+      }
+    }
+  };
   NodePar np{};
   int node = -1;
   bool active = false, want = true, eval_pending = false, post_pending = false, final_step = false;
@@ -173,9 +310,12 @@ __global__ __launch_bounds__(BLOCK, 1) void evolve_kernel(EvolveArgs a) {
   const int n_claim = a.n_claim_dev ? *a.n_claim_dev : a.n_claim;
 
   int phase = 5;
+  double ts = 0.0;   // time of the stage input ytmp (carried over for phases 1-4)
   for (;;) {
+    // Stage input, stage derivative and partial sums: rewritten before every
+    // use within an iteration, so nothing of them is carried round the loop.
+    double ytmp[kNV], kout[kNV], sp[kNV];
     bool do_rhs = false;
-    double ts = 0.0;
     if (phase == 5) {
       // ---- claim nodes for lanes that need one (convergent point) ------
       while (__any(want)) {
@@ -187,6 +327,15 @@ __global__ __launch_bounds__(BLOCK, 1) void evolve_kernel(EvolveArgs a) {
           } else {
             node = idx;
             np = load_node_par(a.np, a.ld, node);
+            if constexpr (WAVES == 2) {
+              XC[(kNV + 1) * BLOCK] = np.t_ref;
+              XC[(kNV + 2) * BLOCK] = np.m_ref;
+              XC[(kNV + 3) * BLOCK] = np.m_rate;
+              XC[(kNV + 4) * BLOCK] = np.spin_rate;
+              XC[(kNV + 5) * BLOCK] = np.ram_force;
+              XC[(kNV + 6) * BLOCK] = np.v_host;
+              XC[(kNV + 7) * BLOCK] = np.satellite ? 1.0 : 0.0;
+            }
             t0 = a.t_start[node];
             t1 = a.t_end[node];
             h_node = a.h_step[node];
@@ -253,46 +402,89 @@ __global__ __launch_bounds__(BLOCK, 1) void evolve_kernel(EvolveArgs a) {
           ++n_attempt;
           ts = t + ck::a2 * h0;
 #pragma unroll
-          for (int v = 0; v < kNV; ++v) ytmp[v] = Y(v) + h0 * (ck::b21 * KS(0, v));
+          for (int c = 0; c < kNV; c += kChunk) {
+            double gy[kChunk], gk[kChunk];
+#pragma unroll
+            for (int i = 0; i < kChunk; ++i) { gy[i] = Y(c + i); gk[i] = KS(0, c + i); }
+#pragma unroll
+            for (int i = 0; i < kChunk; ++i) ytmp[c + i] = gy[i] + h0 * (ck::b21 * gk[i]);
+          }
           break;
         }
-        case 1:
-          ts = t + ck::a3 * h0;
-#pragma unroll
-          for (int v = 0; v < kNV; ++v) ytmp[v] = Y(v) + h0 * (ck::b31 * KS(0, v) + ck::b32 * KS(1, v));
-          break;
-        case 2:
-          ts = t + ck::a4 * h0;
-#pragma unroll
-          for (int v = 0; v < kNV; ++v)
-            ytmp[v] = Y(v) + h0 * (ck::b41 * KS(0, v) + ck::b42 * KS(1, v) + ck::b43 * KS(2, v));
-          break;
-        case 3:
-          ts = t + ck::a5 * h0;
-#pragma unroll
-          for (int v = 0; v < kNV; ++v)
-            ytmp[v] = Y(v) + h0 * (ck::b51 * KS(0, v) + ck::b52 * KS(1, v) + ck::b53 * KS(2, v) +
-                                   ck::b54 * KS(3, v));
-          break;
         default:
-          ts = t + ck::a6 * h0;
+          // phases 1-4: y + h (sum of the older stage terms + the newest)
+          if constexpr (WAVES == 2) {
+            // formed at the end of the previous iteration, in the exchange area
+            ts = XC[kNV * BLOCK];
 #pragma unroll
-          for (int v = 0; v < kNV; ++v)
-            ytmp[v] = Y(v) + h0 * (ck::b61 * KS(0, v) + ck::b62 * KS(1, v) + ck::b63 * KS(2, v) +
-                                   ck::b64 * KS(3, v) + ck::b65 * KS(4, v));
+            for (int v = 0; v < kNV; ++v) ytmp[v] = XC[v * BLOCK];
+          } else {
+            stage_partials(sp, phase - 1, true);
+            const double b_new = stage_b_new(phase - 1);
+            ts = t + stage_a_new(phase - 1) * h0;
+#pragma unroll
+            for (int c = 0; c < kNV; c += kChunk) {
+              double gy[kChunk], gk[kChunk];
+#pragma unroll
+              for (int i = 0; i < kChunk; ++i) { gy[i] = Y(c + i); gk[i] = KS(phase, c + i); }
+#pragma unroll
+              for (int i = 0; i < kChunk; ++i) ytmp[c + i] = gy[i] + h0 * (sp[c + i] + b_new * gk[i]);
+            }
+          }
           break;
       }
     }
 
-    if (do_rhs) {
-      rhs(P, a.D, a.tab, a.cd, mode, np, ts, ytmp, kout);
-      ++n_rhs;
+    if constexpr (WAVES == 2) {
+      const bool any_rhs = __any(do_rhs);
+      if (phase == 0 || phase == 5) {
+#pragma unroll
+        for (int v = 0; v < kNV; ++v) XC[v * BLOCK] = ytmp[v];
+        XC[kNV * BLOCK] = ts;
+      }
+      if (threadIdx.x == 0) xchg_cmd = any_rhs ? kCmdCompute : kCmdSkip;
+      __syncthreads();   // stage state published
+      if (do_rhs) {
+        const UniformRefs u = uniform_refs(P);
+        rhs_part<kRhsGalaxy>(u.P, *u.D, a.tab, *u.C, mode, np, ts, ytmp, kout);
+        ++n_rhs;
+      }
+      stage_partials(sp, phase, active);   // while the hot half is still in flight
+      __syncthreads();   // hot-half derivatives published
+      if (do_rhs) {
+        double kh[kNV];
+#pragma unroll
+        for (int v = 0; v < kNV; ++v) kh[v] = XC[(kNV + 8 + v) * BLOCK];
+#pragma unroll
+        for (int v = 0; v < kNV; ++v) kout[v] += kh[v];
+      }
+    } else {
+      if (do_rhs) {
+        const UniformRefs u = uniform_refs(P);
+        rhs(u.P, *u.D, a.tab, *u.C, mode, np, ts, ytmp, kout);
+        ++n_rhs;
+      }
+      if (phase == 4) stage_partials(sp, phase, active);
     }
 
     if (phase < 4) {
       if (active) {
 #pragma unroll
         for (int v = 0; v < kNV; ++v) kset(phase + 1, v, kout[v]);
+        if constexpr (WAVES == 2) {
+          // the next stage's input, straight into the exchange area (the
+          // other wave is past its reads and waits for the next publish)
+          const double b_new = stage_b_new(phase);
+          XC[kNV * BLOCK] = t + stage_a_new(phase) * h0;
+#pragma unroll
+          for (int c = 0; c < kNV; c += kChunk) {
+            double gy[kChunk];
+#pragma unroll
+            for (int i = 0; i < kChunk; ++i) gy[i] = Y(c + i);
+#pragma unroll
+            for (int i = 0; i < kChunk; ++i) XC[(c + i) * BLOCK] = gy[i] + h0 * (sp[c + i] + b_new * kout[c + i]);
+          }
+        }
       }
       ++phase;
       continue;
@@ -307,7 +499,13 @@ __global__ __launch_bounds__(BLOCK, 1) void evolve_kernel(EvolveArgs a) {
           post_pending = false;
         }
       }
-      if (!__any(active)) break;
+      if (!__any(active)) {
+        if constexpr (WAVES == 2) {
+          if (threadIdx.x == 0) xchg_cmd = kCmdQuit;
+          __syncthreads();
+        }
+        break;
+      }
       phase = 0;
       continue;
     }
@@ -317,12 +515,22 @@ __global__ __launch_bounds__(BLOCK, 1) void evolve_kernel(EvolveArgs a) {
     if (!active) continue;
     ErrorNorm en;
 #pragma unroll
-    for (int v = 0; v < kNV; ++v) {
-      const double k1v = KS(0, v), k3v = KS(2, v), k4v = KS(3, v), k5v = KS(4, v);
-      const double yn = Y(v) + h0 * (ck::c1 * k1v + ck::c3 * k3v + ck::c4 * k4v + ck::c6 * kout[v]);
-      const double ye = h0 * (ck::ec1 * k1v + ck::ec3 * k3v + ck::ec4 * k4v + ck::ec5 * k5v + ck::ec6 * kout[v]);
-      ytmp[v] = yn;
-      error_norm_add(en, a.eps_abs, a.eps_rel, 1.0, 0.0, h0, yn, ye, 0.0, SC(v), enforce && var_non_negative(v));
+    for (int c = 0; c < kNV; c += kChunk) {
+      double gy[kChunk], g1[kChunk], g3[kChunk], g4[kChunk], g5[kChunk], gs[kChunk];
+#pragma unroll
+      for (int i = 0; i < kChunk; ++i) {
+        gy[i] = Y(c + i); g1[i] = KS(0, c + i); g3[i] = KS(2, c + i); g4[i] = KS(3, c + i); g5[i] = KS(4, c + i);
+        gs[i] = SC(c + i);
+      }
+#pragma unroll
+      for (int i = 0; i < kChunk; ++i) {
+        const int v = c + i;
+        const double yn = gy[i] + h0 * (sp[v] + ck::c6 * kout[v]);
+        const double ye = h0 * (ck::ec1 * g1[i] + ck::ec3 * g3[i] + ck::ec4 * g4[i] + ck::ec5 * g5[i] +
+                                ck::ec6 * kout[v]);
+        ytmp[v] = yn;
+        error_norm_add(en, a.eps_abs, a.eps_rel, 1.0, 0.0, h0, yn, ye, 0.0, gs[i], enforce && var_non_negative(v));
+      }
     }
     double h_adj = h0;
     const int decision = hadjust_decide(en, kRkckOrder, h_adj);
@@ -480,9 +688,9 @@ __global__ void cost_probe_kernel(int n_items, const int* __restrict__ subset, i
   index[j] = i;
 }
 
-template <int BLOCK, bool YLDS, bool KLDS>
+template <int BLOCK, bool YLDS, bool KLDS, int WAVES = 1>
 constexpr size_t lds_bytes() {
-  return sizeof(double) * ((YLDS ? 2 : 0) + (KLDS ? kStages : 0)) * kNV * BLOCK;
+  return lds_vector_bytes<BLOCK, YLDS, KLDS>() + (WAVES == 2 ? sizeof(double) * kXchgVectors * BLOCK : 0);
 }
 
 }  // namespace
@@ -619,7 +827,15 @@ int gal_ode_batch_create(int n_max, double tol_abs, double tol_rel, int flags, g
     static_assert(lds_bytes<64, true, true>() <= 163840, "lane-private LDS vectors must fit 160 KiB");
     static_assert(2 * lds_bytes<64, false, true>() <= 163840, "two workgroups per CU must fit 160 KiB");
     static_assert(4 * lds_bytes<64, true, false>() <= 163840, "four workgroups per CU must fit 160 KiB");
-    if (hipFuncSetAttribute((const void*)evolve_kernel<64, true, true>, hipFuncAttributeMaxDynamicSharedMemorySize,
+    static_assert(lds_bytes<64, true, true, 2>() + 16 <= 163840, "two-wave groups with LDS stages must fit 160 KiB");
+    static_assert(2 * (lds_bytes<64, true, false, 2>() + 16) <= 163840, "two two-wave groups per CU must fit");
+    if (hipFuncSetAttribute((const void*)evolve_kernel<64, true, true, 2>,
+                            hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds_bytes<64, true, true, 2>()) !=
+            hipSuccess ||
+        hipFuncSetAttribute((const void*)evolve_kernel<64, true, false, 2>,
+                            hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds_bytes<64, true, false, 2>()) !=
+            hipSuccess ||
+        hipFuncSetAttribute((const void*)evolve_kernel<64, true, true>, hipFuncAttributeMaxDynamicSharedMemorySize,
                             (int)lds_bytes<64, true, true>()) != hipSuccess ||
         hipFuncSetAttribute((const void*)evolve_kernel<64, false, true>, hipFuncAttributeMaxDynamicSharedMemorySize,
                             (int)lds_bytes<64, false, true>()) != hipSuccess ||
@@ -724,9 +940,40 @@ int gal_ode_batch_set_launch(gal_ode_batch* h, int nodes_per_block, int blocks_p
   return GAL_OK;
 }
 
+int gal_ode_batch_set_rhs_waves(gal_ode_batch* h, int waves) {
+  if (!h || (waves != 1 && waves != 2)) return GAL_E_INVALID;
+  h->rhs_waves = waves;
+  return GAL_OK;
+}
+
+int gal_ode_batch_get_launch(gal_ode_batch* h, int* nodes_per_block, int* blocks_per_cu, int* rhs_waves) {
+  if (!h) return GAL_E_INVALID;
+  if (nodes_per_block) *nodes_per_block = h->nodes_per_block;
+  if (blocks_per_cu) *blocks_per_cu = h->blocks_per_cu;
+  if (rhs_waves) *rhs_waves = h->rhs_waves;
+  return GAL_OK;
+}
+
 static int launch_evolve(gal_ode_batch* h, const EvolveArgs& args, hipStream_t s) {
   const int blocks = std::max(1, h->n_cus * h->blocks_per_cu);
+  if (h->rhs_waves == 2 && h->blocks_per_cu > 2) return GAL_E_INVALID;
   hipLaunchKernelGGL(reset_counter_kernel, dim3(1), dim3(64), 0, s, args.work);
+  if (h->rhs_waves == 2) {
+    // two waves per group of 64 nodes: one group per CU with the stage
+    // derivatives in LDS, or two with them in the HBM workspace
+    if (h->blocks_per_cu == 1) {
+      constexpr size_t lds = lds_bytes<64, true, true, 2>();
+      hipLaunchKernelGGL((evolve_kernel<64, true, true, 2>), dim3(blocks), dim3(128), lds, s, args);
+    } else {
+      EvolveArgs b = args;
+      b.kwork = h->d_kwork;
+      if (blocks * 64 > h->kwork_lanes) return GAL_E_INVALID;
+      constexpr size_t lds = lds_bytes<64, true, false, 2>();
+      hipLaunchKernelGGL((evolve_kernel<64, true, false, 2>), dim3(blocks), dim3(128), lds, s, b);
+    }
+    HIP_TRY(hipGetLastError());
+    return GAL_OK;
+  }
   constexpr size_t lds_all = lds_bytes<64, true, true>(), lds_k = lds_bytes<64, false, true>(),
                    lds_y = lds_bytes<64, true, false>();
   if (h->blocks_per_cu == 1) {
@@ -871,15 +1118,17 @@ int gal_ode_batch_autotune(gal_ode_batch* h, int n, int ld, double* y, const dou
     hipFree(wy); hipFree(wh); hipFree(wsc); hipFree(wst);
     return GAL_E_NOMEM;
   }
-  const int shapes[][2] = {{64, 1}, {64, 2}, {64, 4}};
+  // (nodes per workgroup, workgroups per CU, waves per group of nodes)
+  const int shapes[][3] = {{64, 1, 1}, {64, 2, 1}, {64, 4, 1}, {64, 1, 2}, {64, 2, 2}};
   float best = 1e30f;
-  int bn = h->nodes_per_block, bb = h->blocks_per_cu;
+  int bn = h->nodes_per_block, bb = h->blocks_per_cu, bw = h->rhs_waves;
   hipEvent_t e0, e1;
   hipEventCreate(&e0);
   hipEventCreate(&e1);
   int rc = GAL_OK;
   for (auto& sh : shapes) {
-    if (gal_ode_batch_set_launch(h, sh[0], sh[1]) != GAL_OK) continue;
+    if (gal_ode_batch_set_launch(h, sh[0], sh[1]) != GAL_OK || gal_ode_batch_set_rhs_waves(h, sh[2]) != GAL_OK)
+      continue;
     for (int rep = 0; rep < 2 && rc == GAL_OK; ++rep) {
       hipMemcpyAsync(wy, y, sizeof(double) * GAL_NVAR * nn, hipMemcpyDeviceToDevice, s);
       hipMemcpyAsync(wh, h_step, sizeof(double) * nn, hipMemcpyDeviceToDevice, s);
@@ -895,6 +1144,7 @@ int gal_ode_batch_autotune(gal_ode_batch* h, int n, int ld, double* y, const dou
         best = ms;
         bn = sh[0];
         bb = sh[1];
+        bw = sh[2];
       }
     }
   }
@@ -903,6 +1153,7 @@ int gal_ode_batch_autotune(gal_ode_batch* h, int n, int ld, double* y, const dou
   hipFree(wy); hipFree(wh); hipFree(wsc); hipFree(wst);
   (void)status;
   gal_ode_batch_set_launch(h, bn, bb);
+  gal_ode_batch_set_rhs_waves(h, bw);
   if (best_nodes_per_block) *best_nodes_per_block = bn;
   if (best_blocks_per_cu) *best_blocks_per_cu = bb;
   return rc;

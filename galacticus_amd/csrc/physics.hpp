@@ -231,19 +231,39 @@ __device__ __forceinline__ Halo halo(const double* __restrict__ P, const Derived
 // chains (halo scales, ISCO, metallicity, star formation, hot-halo
 // structure) can be interleaved by the scheduler.  A rate that does not
 // apply is exactly zero, which leaves the accumulated sums unchanged.
-__device__ __forceinline__ void rhs(const double* __restrict__ P, const DerivedConstants& D,
-                                    const double* __restrict__ tab, const CoolingDesc& cd, int mode,
-                                    const NodePar& np, double t, const double (&y)[kNV], double (&f)[kNV]) {
+//
+// The operators fall into two halves that share only the halo scales and
+// the state: the galaxy half (spin growth, satellite mass loss, star
+// formation, stellar feedback, bar instability, CGM accretion and outflow
+// reincorporation) and the hot-halo half (ISCO, hot-halo structure, cooling
+// function and radius, black-hole accretion, jet and winds, cooling and
+// heating, CGM outer radius, radiated energy).  rhs_part<PART> computes one
+// half (or both); every path forms the derivative as f_galaxy + f_halo, so a
+// wave that computes one half while another wave computes the other gets the
+// same bits as one lane computing both.
+enum : int { kRhsGalaxy = 1, kRhsHalo = 2 };
+
+template <int PART>
+__device__ __forceinline__ void rhs_part(const double* __restrict__ P, const DerivedConstants& D,
+                                         const double* __restrict__ tab, const CoolingDesc& cd, int mode,
+                                         const NodePar& np, double t, const double (&y)[kNV], double (&f)[kNV]) {
+  constexpr bool GAL = (PART & kRhsGalaxy) != 0, HOT = (PART & kRhsHalo) != 0;
 #pragma unroll
   for (int i = 0; i < kNV; ++i) f[i] = 0.0;
   if (mode == GAL_PHYSICS_TEST) {
     // source/tests/ODE_solver/functions.F90:37-80
-    f[0] = sin(t);
-    f[1] = y[2];
-    f[2] = -1.0 * y[1];
+    if constexpr (GAL) {
+      f[0] = sin(t);
+      f[1] = y[2];
+      f[2] = -1.0 * y[1];
+    }
     return;
   }
   const bool box = mode == GAL_PHYSICS_BOX;
+  // closed-box mode has no hot halo, black hole or CGM
+  if constexpr (!GAL) {
+    if (box) return;
+  }
   // ---- chains that do not depend on the halo scales ---------------------
   const double m_bh = y[GAL_V_BH_MASS];
   const double spin = fmin(fmax(y[GAL_V_BH_SPIN], 0.0), 0.998);
@@ -284,9 +304,10 @@ __device__ __forceinline__ void rhs(const double* __restrict__ P, const DerivedC
   const double t_vir = D.t_vir_per_v2 * H.v_vir * H.v_vir;
   const double lt = log(clamp_temperature(cd, t_vir));
 
+  const int ops = box ? 0 : (int)P[GAL_P_OPERATORS];
+  if constexpr (GAL) {
   // spin%angularMomentum growth (halo_angular_momentum_interpolate.F90)
   f[GAL_V_SPIN_ANGULAR_MOMENTUM] += np.satellite ? 0.0 : np.spin_rate;
-  const int ops = box ? 0 : (int)P[GAL_P_OPERATORS];
   // satelliteMassLoss (satellite/mass_loss/_class.F90:176-200): an isolated
   // halo's bound mass follows its basic mass; darkMatterHaloMassLossRate is
   // the default zero class for satellites.
@@ -424,6 +445,42 @@ __device__ __forceinline__ void rhs(const double* __restrict__ P, const DerivedC
     f[GAL_V_DISK_ABUNDANCES_STELLAR] -= rzs;
     f[GAL_V_SPHEROID_ABUNDANCES_STELLAR] += rzs;
   }
+  // ---- CGMAccretion (circumgalactic_medium/accretion.F90:517) with
+  //      accretionHalo simple (accretion/halo/simple.F90:281-378) ---------
+  {
+    const bool acc = !np.satellite && H.mass > 0.0;
+    const double failed = (H.a3 > D.a3_reion && H.v_vir < P[GAL_P_VELOCITY_SUPPRESSION_REIONIZATION]) ? 1.0 : 0.0;
+    const double inv_mass = rcp(H.mass);
+    const double growth = np.m_rate * inv_mass;
+    const double m_un = y[GAL_V_HOT_UNACCRETED_MASS], z_un = y[GAL_V_HOT_UNACCRETED_ABUNDANCES];
+    const double rate_acc = D.f_baryon * np.m_rate * (1.0 - failed) + m_un * growth * (1.0 - failed);
+    const double rate_fail = D.f_baryon * np.m_rate * failed - m_un * growth * (1.0 - failed);
+    const double rate_z = z_un * growth * (1.0 - failed);
+    f[GAL_V_HOT_MASS] += acc ? rate_acc : 0.0;
+    f[GAL_V_HOT_UNACCRETED_MASS] += acc ? rate_fail : 0.0;
+    f[GAL_V_HOT_ABUNDANCES] += acc ? rate_z : 0.0;
+    f[GAL_V_HOT_UNACCRETED_ABUNDANCES] -= acc ? rate_z : 0.0;
+    f[GAL_V_HOT_ANGULAR_MOMENTUM] += acc ? rate_acc * (y[GAL_V_SPIN_ANGULAR_MOMENTUM] * inv_mass) : 0.0;
+  }
+  // ---- CGMOutflowReincorporation (outflow_reincorporation.F90:272-330)
+  //      haloDynamicalTimeRate (halo_dynamical_time.F90:113-125) ----------
+  {
+    const double m_o = y[GAL_V_HOT_OUTFLOWED_MASS];
+    const bool ok = m_o > 0.0;
+    const double q = P[GAL_P_REINCORPORATION_MULTIPLIER] * H.inv_t_dyn;
+    const double r = m_o * q;
+    const double rr = ok ? r : 0.0;
+    const double rj = ok ? y[GAL_V_HOT_OUTFLOWED_ANGULAR_MOMENTUM] * q : 0.0;
+    const double rz = ok ? y[GAL_V_HOT_OUTFLOWED_ABUNDANCES] * q : 0.0;
+    f[GAL_V_HOT_OUTFLOWED_MASS] -= rr;
+    f[GAL_V_HOT_OUTFLOWED_ANGULAR_MOMENTUM] -= rj;
+    f[GAL_V_HOT_OUTFLOWED_ABUNDANCES] -= rz;
+    f[GAL_V_HOT_MASS] += rr;
+    f[GAL_V_HOT_ANGULAR_MOMENTUM] += rj;
+    f[GAL_V_HOT_ABUNDANCES] += rz;
+  }
+  }  // galaxy half
+  if constexpr (!HOT) return;
   // ---- hot-halo structure and cooling radius ----------------------------
   // beta=2/3 profile, core coreRadiusOverVirialRadius*r_vir, truncated at
   // outerRadius (hot_halo/mass_distribution/beta_profile.F90); virial
@@ -554,40 +611,6 @@ __device__ __forceinline__ void rhs(const double* __restrict__ P, const DerivedC
     f[GAL_V_HOT_OUTFLOWED_ANGULAR_MOMENTUM] += jo * D.inv_one_minus_j_loss;
     f[GAL_V_HOT_OUTFLOWED_ABUNDANCES] += zo;
   }
-  // ---- CGMAccretion (circumgalactic_medium/accretion.F90:517) with
-  //      accretionHalo simple (accretion/halo/simple.F90:281-378) ---------
-  {
-    const bool acc = !np.satellite && H.mass > 0.0;
-    const double failed = (H.a3 > D.a3_reion && H.v_vir < P[GAL_P_VELOCITY_SUPPRESSION_REIONIZATION]) ? 1.0 : 0.0;
-    const double inv_mass = rcp(H.mass);
-    const double growth = np.m_rate * inv_mass;
-    const double m_un = y[GAL_V_HOT_UNACCRETED_MASS], z_un = y[GAL_V_HOT_UNACCRETED_ABUNDANCES];
-    const double rate_acc = D.f_baryon * np.m_rate * (1.0 - failed) + m_un * growth * (1.0 - failed);
-    const double rate_fail = D.f_baryon * np.m_rate * failed - m_un * growth * (1.0 - failed);
-    const double rate_z = z_un * growth * (1.0 - failed);
-    f[GAL_V_HOT_MASS] += acc ? rate_acc : 0.0;
-    f[GAL_V_HOT_UNACCRETED_MASS] += acc ? rate_fail : 0.0;
-    f[GAL_V_HOT_ABUNDANCES] += acc ? rate_z : 0.0;
-    f[GAL_V_HOT_UNACCRETED_ABUNDANCES] -= acc ? rate_z : 0.0;
-    f[GAL_V_HOT_ANGULAR_MOMENTUM] += acc ? rate_acc * (y[GAL_V_SPIN_ANGULAR_MOMENTUM] * inv_mass) : 0.0;
-  }
-  // ---- CGMOutflowReincorporation (outflow_reincorporation.F90:272-330)
-  //      haloDynamicalTimeRate (halo_dynamical_time.F90:113-125) ----------
-  {
-    const double m_o = y[GAL_V_HOT_OUTFLOWED_MASS];
-    const bool ok = m_o > 0.0;
-    const double q = P[GAL_P_REINCORPORATION_MULTIPLIER] * H.inv_t_dyn;
-    const double r = m_o * q;
-    const double rr = ok ? r : 0.0;
-    const double rj = ok ? y[GAL_V_HOT_OUTFLOWED_ANGULAR_MOMENTUM] * q : 0.0;
-    const double rz = ok ? y[GAL_V_HOT_OUTFLOWED_ABUNDANCES] * q : 0.0;
-    f[GAL_V_HOT_OUTFLOWED_MASS] -= rr;
-    f[GAL_V_HOT_OUTFLOWED_ANGULAR_MOMENTUM] -= rj;
-    f[GAL_V_HOT_OUTFLOWED_ABUNDANCES] -= rz;
-    f[GAL_V_HOT_MASS] += rr;
-    f[GAL_V_HOT_ANGULAR_MOMENTUM] += rj;
-    f[GAL_V_HOT_ABUNDANCES] += rz;
-  }
   // ---- CGMCoolingHeating (circumgalactic_medium/cooling_heating.F90:216-381)
   //      whiteFrenk1991Rate (cooling_rate/White-Frenk.F90:132-185) ---------
   {
@@ -674,6 +697,17 @@ __device__ __forceinline__ void rhs(const double* __restrict__ P, const DerivedC
     const double count = nh * nall_per_nh * (4.0 * kPi / 3.0) * rcm * rcm * rcm;
     f[GAL_V_HOT_ENERGY_RADIATED] += hot_ok ? lambda_cool * nh * count : 0.0;
   }
+}
+
+// The whole RHS: both halves, summed as f_galaxy + f_halo.
+__device__ __forceinline__ void rhs(const double* __restrict__ P, const DerivedConstants& D,
+                                    const double* __restrict__ tab, const CoolingDesc& cd, int mode,
+                                    const NodePar& np, double t, const double (&y)[kNV], double (&f)[kNV]) {
+  double fh[kNV];
+  rhs_part<kRhsGalaxy>(P, D, tab, cd, mode, np, t, y, f);
+  rhs_part<kRhsHalo>(P, D, tab, cd, mode, np, t, y, fh);
+#pragma unroll
+  for (int i = 0; i < kNV; ++i) f[i] += fh[i];
 }
 
 // Component scale-set rules at the start of an evolve (hot_halo/standard/_class.F90:804-840,

@@ -100,9 +100,20 @@ class NodeEvolverBatched:
             "gal_ode_batch_set_cooling_table")
         self.table = t
 
-    def set_launch(self, nodes_per_block: int, blocks_per_cu: int):
+    def set_launch(self, nodes_per_block: int, blocks_per_cu: int, rhs_waves: Optional[int] = None):
+        """Launch shape: nodes per workgroup, workgroups per CU and, if given,
+        waves per group of nodes (2 splits every RHS over two SIMDs)."""
         _lib.check(self.L.gal_ode_batch_set_launch(self.handle, nodes_per_block, blocks_per_cu),
                    "gal_ode_batch_set_launch")
+        if rhs_waves is not None:
+            _lib.check(self.L.gal_ode_batch_set_rhs_waves(self.handle, int(rhs_waves)), "gal_ode_batch_set_rhs_waves")
+
+    def launch(self):
+        """(nodes per workgroup, workgroups per CU, waves per group of nodes)."""
+        bn, bb, bw = ctypes.c_int(), ctypes.c_int(), ctypes.c_int()
+        _lib.check(self.L.gal_ode_batch_get_launch(self.handle, ctypes.byref(bn), ctypes.byref(bb), ctypes.byref(bw)),
+                   "gal_ode_batch_get_launch")
+        return bn.value, bb.value, bw.value
 
     # -- evolution ---------------------------------------------------------
     @staticmethod
@@ -184,7 +195,7 @@ class NodeEvolverBatched:
                                                  _ptr(t_start), _ptr(t_end), _ptr(h_step), _ptr(scale),
                                                  _ptr(status), s, ctypes.byref(bn), ctypes.byref(bb)),
                    "gal_ode_batch_autotune")
-        return bn.value, bb.value
+        return self.launch()
 
     def isAccurate(self, valueNode, valueExpected):
         """standardIsAccurate (node_evolver/standard.F90:757-768)."""

@@ -1,8 +1,10 @@
 """Static algorithmic FLOP count of the device RHS and of one Cash-Karp attempt.
 
-The count is taken from the source text of ``rhs()`` in
-``galacticus_amd/csrc/physics.hpp`` (and the inlined helpers it calls), so it
-moves with the code.  Weights:
+The count is taken from the source text of ``rhs_part()`` in
+``galacticus_amd/csrc/physics.hpp`` (both halves of the RHS, plus the sum
+that joins them, and the inlined helpers), so it moves with the code.  The
+halo scales, which a two-wave launch computes once per half, count once.
+Weights:
 
 * ``+ - * /`` on doubles, ``sqrt``, ``rsqrt``, ``fmin``, ``fmax``, ``fabs``: 1
 * ``exp``, ``log``, ``log1p``, ``cbrt``, ``atan``, ``pow``, ``sinh``: 20 each
@@ -67,7 +69,8 @@ def _count_ops(body: str) -> int:
 
 def rhs_flops() -> int:
     src = open(PHYSICS, encoding="utf-8").read()
-    total = _count_ops(_function_body(src, "__device__ __forceinline__ void rhs("))
+    total = _count_ops(_function_body(src, "__device__ __forceinline__ void rhs_part("))
+    total += NVAR    # f = f_galaxy + f_halo
     # helpers inlined once per evaluation
     total += _count_ops(_function_body(src, "__device__ __forceinline__ Halo halo("))
     total += _count_ops(_function_body(src, "__device__ __forceinline__ void isco("))

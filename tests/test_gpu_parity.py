@@ -197,24 +197,31 @@ def test_evolve_matches_oracle(torch_cuda, cooling_table, reuse):
 
 def test_launch_variants_bit_identical(torch_cuda, cooling_table):
     """One workgroup per CU (every lane vector in LDS), two (state in the HBM
-    workspace) and four (stage derivatives in the HBM workspace) run the same
-    arithmetic: identical results, bit for bit."""
+    workspace) and four (stage derivatives in the HBM workspace), each with
+    one wave per group of nodes, and one or two two-wave groups per CU (the
+    RHS split over two SIMDs) run the same arithmetic: identical results,
+    step sizes, statuses and counters, bit for bit."""
     torch = torch_cuda
     physics = gp.physics_params()
     b = wl.isolated_halos(4000, seed=21, params=physics)
     ev = _evolver(b.n, physics)
     d = lambda a: torch.from_numpy(np.ascontiguousarray(a)).cuda()
     out = []
-    for shape in ((64, 1), (64, 2), (64, 4)):
+    shapes = ((64, 1, 1), (64, 2, 1), (64, 4, 1), (64, 1, 2), (64, 2, 2))
+    for shape in shapes:
+        print("launch", shape, flush=True)
         ev.set_launch(*shape)
+        assert ev.launch() == shape
         y, h = d(b.y), d(b.h_step)
-        st = ev.evolve_device(y, d(b.node_par), d(b.t_start), d(b.t_end), h)
+        cnt = torch.zeros(3 * b.n, dtype=torch.int64, device="cuda")
+        st = ev.evolve_device(y, d(b.node_par), d(b.t_start), d(b.t_end), h, counters=cnt)
         torch.cuda.synchronize()
-        out.append((y.cpu(), h.cpu(), st.cpu()))
+        out.append((y.cpu(), h.cpu(), st.cpu(), cnt.cpu()))
+    assert (out[0][2] == 0).all()
     for o in out[1:]:
-        assert torch.equal(out[0][0], o[0]) and torch.equal(out[0][1], o[1]) and torch.equal(out[0][2], o[2])
+        assert all(torch.equal(r, g) for r, g in zip(out[0], o))
     shape = ev.autotune(d(b.y), d(b.node_par), d(b.t_start), d(b.t_end), d(b.h_step))
-    assert shape in ((64, 1), (64, 2), (64, 4))
+    assert shape in shapes
 
 
 def test_evolve_captured_in_a_hip_graph(torch_cuda, cooling_table):
