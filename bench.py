@@ -33,8 +33,10 @@ sys.path.insert(0, ROOT)
 # concurrent forests): with 8 queues the four forest streams shared queues
 # (64 trees in 0.29 s), with 16 they did not (0.17 s).  Read when the HIP
 # runtime starts, so it is set before torch is.
-if int(os.environ.get("GPU_MAX_HW_QUEUES", "4")) < 16:
-    os.environ["GPU_MAX_HW_QUEUES"] = "16"
+# Up to 32 evolve streams: 32 queues (the most this pool allows) keep every
+# one of 16 on a queue of its own and let 32 share them in pairs.
+if int(os.environ.get("GPU_MAX_HW_QUEUES", "4")) < 32:
+    os.environ["GPU_MAX_HW_QUEUES"] = "32"
 
 METRIC = "node-Gyr evolved/sec: FP64 batched RK vs MLP+fallback at ≤1% M* error"
 FP64_PEAK_TFLOPS = 78.6
@@ -45,8 +47,8 @@ FP64_PEAK_TFLOPS = 78.6
 # counter scaled by what it reads for one elementwise kernel that reads and
 # writes 1 GiB (FETCH_SIZE x 2.0000, WRITE_SIZE x 1.0000).
 HBM_TRAFFIC_1E5 = {
-    (64, 1): (228962.6 * 2.0 + 218809.2 * 1.0) * 1024,
-    (64, 2): (330335.9 * 2.0 + 707914.9 * 1.0) * 1024,
+    (64, 1, 2): (241539.5 * 2.0 + 226578.7 * 1.0) * 1024,
+    (64, 2, 2): (1884151.2 * 2.0 + 3845819.0 * 1.0) * 1024,
 }
 
 
@@ -591,10 +593,11 @@ def main():
     ap.add_argument("--warmup", type=int, default=4)
     ap.add_argument("--nodes", type=int, default=100000)
     ap.add_argument("--seed", type=int, default=0)
-    ap.add_argument("--streams", type=int, default=4,
-                    help="independent batches alternate over this many streams and evolver handles")
+    ap.add_argument("--streams", default="auto",
+                    help="independent batches alternate over this many streams and evolver handles, or 'auto' "
+                         "for one per timed step (up to 32)")
     ap.add_argument("--launch", default="auto",
-                    help="launch shape 'nodes_per_workgroup,workgroups_per_cu', or 'auto' for the autotuner")
+                    help="launch shape 'nodes_per_workgroup,workgroups_per_cu[,rhs_waves]', or 'auto' to time them")
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
     ap.add_argument("--cpu-seconds", type=float, default=15.0)
     ap.add_argument("--no-surrogate", action="store_true", help="skip the surrogate training/inference legs")
@@ -640,7 +643,10 @@ def main():
     batch = wl.isolated_halos(args.nodes, seed=args.seed, params=physics)
     n = batch.n
     node_gyr = batch.node_gyr
-    n_streams = max(1, args.streams)
+    # Every timed batch in flight at once (up to 32): each batch's tail -- the
+    # few stiffest nodes, whose chains set the launch time -- then overlaps
+    # the others instead of queueing behind a batch on the same stream.
+    n_streams = min(max(1, args.steps), 32) if args.streams == "auto" else max(1, int(args.streams))
     evs = [NodeEvolverBatched(n, physics, table, odeToleranceAbsolute=tol_abs, odeToleranceRelative=tol_rel,
                               reuseODEStepSize=True) for _ in range(n_streams)]
     ev = evs[0]
@@ -660,39 +666,42 @@ def main():
     # share the same queues.
     streams = [torch.cuda.Stream() for _ in range(n_streams)]
     tuning = {}
+    shapes = ((64, 1, 1), (64, 2, 1), (64, 4, 1), (64, 1, 2), (64, 2, 2))
+    if args.launch != "auto":
+        v = tuple(int(x) for x in args.launch.split(","))
+        shapes = (v if len(v) == 3 else v + (1,),)
     if args.launch == "auto":
-        # The library's autotuner times single launches; the timed region
-        # overlaps launches on several streams, where the shape with more
-        # resident lanes can win although it loses alone.  Time each shape
-        # both ways and keep the faster in the overlapped regime.
         single = ev.autotune(y0, npar, t0, t1, h0)
         tuning["single_launch_choice"] = list(single)
+    if len(shapes) > 1:
+        # The library's autotuner times single launches; the timed region
+        # overlaps independent batches on n_streams streams, where a shape
+        # with more resident lanes can win although it loses alone.  Time
+        # each shape on one round of n_streams concurrent batches and keep
+        # the fastest per batch.
         best = None
-        tune_streams = streams
-        reps = 6 * len(evs)
-        for shape in ((64, 1), (64, 2), (64, 4), (64, 1), (64, 2)):
+        for shape in shapes:
             for e in evs:
                 e.set_launch(*shape)
-            yt = [y0.clone() for _ in range(reps)]
-            ht = [h0.clone() for _ in range(reps)]
+            yt = [y0.clone() for _ in range(n_streams)]
+            ht = [h0.clone() for _ in range(n_streams)]
             torch.cuda.synchronize()
             t_a = time.perf_counter()
-            for k in range(reps):
-                j = k % len(evs)
-                evs[j].evolve_device(yt[k], npar, t0, t1, ht[k], stream=tune_streams[j])
+            for j in range(n_streams):
+                evs[j].evolve_device(yt[j], npar, t0, t1, ht[j], stream=streams[j])
             torch.cuda.synchronize()
-            ms = (time.perf_counter() - t_a) * 1e3 / reps
-            key = f"{shape[0]},{shape[1]}"
-            tuning[key] = min(ms, tuning.get(key, ms))
+            ms = (time.perf_counter() - t_a) * 1e3 / n_streams
+            tuning[f"{shape[0]},{shape[1]},{shape[2]}"] = ms
             if best is None or ms < best[0]:
                 best = (ms, shape)
             del yt, ht
         launch = best[1]
     else:
-        launch = tuple(int(v) for v in args.launch.split(","))
+        launch = shapes[0]
     for e in evs:
         e.set_launch(*launch)
-    log(f"launch: {launch[0]} nodes per workgroup, {launch[1]} workgroups per CU (tuning {tuning})")
+    log(f"launch: {launch[0]} nodes per workgroup, {launch[1]} workgroups per CU, {launch[2]} waves per group, "
+        f"{n_streams} streams (tuning {tuning})")
     total = args.warmup + args.steps
     ys = [y0.clone() for _ in range(total)]
     hs = [h0.clone() for _ in range(total)]
@@ -778,8 +787,11 @@ def main():
             "workload": "batched plain RK: isolated-halo node ODE systems (FP64 Cash-Karp, rel 1e-3, abs 1e-6 x scale)",
             "n_nodes": n, "n_vars": 30, "node_gyr": node_gyr, "seed": args.seed,
             "halo_mass": "log-uniform 1e9-1e13 Msun", "f_hot": "uniform 0.05-0.17", "metallicity": "log-uniform 1e-4-0.02",
-            "dt_gyr": "uniform 0.1-2", "cooling_table": "200x50 FP64", "parallelism": "single GPU, one node per lane",
-            "launch": {"nodes_per_workgroup": launch[0], "workgroups_per_cu": launch[1], "streams": n_streams,
+            "dt_gyr": "uniform 0.1-2", "cooling_table": "200x50 FP64", "parallelism": ("single GPU, one node per lane"
+                                                                   + (", the RHS split over two waves"
+                                                                      if launch[2] == 2 else "")),
+            "launch": {"nodes_per_workgroup": launch[0], "workgroups_per_cu": launch[1], "rhs_waves": launch[2],
+                       "streams": n_streams,
                        "tuning_ms_per_step": tuning},
         },
         "single_launch": {
