@@ -17,7 +17,7 @@ struct gal_ode_batch {
   int n_cus = 0;
   int nodes_per_block = 64;
   int blocks_per_cu = 1;
-  int rhs_waves = 1;          // waves per group of 64 nodes (2: the RHS split over two SIMDs)
+  int rhs_waves = 2;          // waves per group of 64 nodes (2: the RHS split over two SIMDs)
   double params[GAL_NPARAM] = {};
   gal::DerivedConstants D{};
   gal::CoolingDesc cd{};
