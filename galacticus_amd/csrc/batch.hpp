@@ -18,6 +18,8 @@ struct gal_ode_batch {
   int nodes_per_block = 64;
   int blocks_per_cu = 1;
   int rhs_waves = 2;          // waves per group of 64 nodes (2: the RHS split over two SIMDs)
+  double wall_limit_s = 0.0;  // per-launch wall-time limit (0: none)
+  double wall_clock_hz = 0.0; // frequency of the device's realtime counter
   double params[GAL_NPARAM] = {};
   gal::DerivedConstants D{};
   gal::CoolingDesc cd{};

@@ -81,6 +81,10 @@ struct EvolveArgs {
   CoolingDesc cd;
   double eps_abs, eps_rel;
   int flags, mode;
+  // Wall-time limit of the launch in ticks of the GPU's constant-rate
+  // realtime counter (0: none).  An attempt that would start after it
+  // returns the node with errorStatusXCPU and its state untouched.
+  unsigned long long wall_ticks;
 };
 
 // Claim one node index per requesting lane with a single atomic per wave.
@@ -311,6 +315,7 @@ __global__ __launch_bounds__(BLOCK * WAVES, 1) void evolve_kernel(EvolveArgs a) 
 
   int phase = 5;
   double ts = 0.0;   // time of the stage input ytmp (carried over for phases 1-4)
+  const unsigned long long t_launch = a.wall_ticks ? wall_clock64() : 0ull;
   for (;;) {
     // Stage input, stage derivative and partial sums: rewritten before every
     // use within an iteration, so nothing of them is carried round the loop.
@@ -392,6 +397,17 @@ __global__ __launch_bounds__(BLOCK * WAVES, 1) void evolve_kernel(EvolveArgs a) 
       do_rhs = true;
       switch (phase) {
         case 0: {
+          // Wall-time limit (node_evolver/standard.F90:693-705 and the RHS's
+          // own check, standard.F90:860-866): the node returns
+          // errorStatusXCPU with its state as it came in.
+          if (a.wall_ticks && wall_clock64() - t_launch > a.wall_ticks) {
+            a.status[node] = GAL_STATUS_XCPU;
+            write_counters(a, node, n_attempt, n_accept, n_rhs);
+            active = false;
+            want = true;
+            do_rhs = false;
+            break;
+          }
           // gsl_odeiv2_evolve_apply: clip the step to reach t1 exactly
           const double dt = t1 - t;
           final_step = false;
@@ -798,6 +814,13 @@ int gal_ode_batch_create(int n_max, double tol_abs, double tol_rel, int flags, g
     return GAL_E_HIP;
   }
   h->n_cus = ncu;
+  {
+    int khz = 0;
+    if (hipDeviceGetAttribute(&khz, hipDeviceAttributeWallClockRate, h->device) == hipSuccess && khz > 0)
+      h->wall_clock_hz = 1.0e3 * khz;
+    else
+      (void)hipGetLastError();
+  }
   const size_t nm = (size_t)std::max(n_max, 1);
   if (hipMalloc(&h->d_params, sizeof(double) * GAL_NPARAM) != hipSuccess ||
       hipMalloc(&h->d_table, sizeof(double) * GAL_COOLING_NT_MAX * GAL_COOLING_NZ_MAX) != hipSuccess ||
@@ -940,6 +963,13 @@ int gal_ode_batch_set_launch(gal_ode_batch* h, int nodes_per_block, int blocks_p
   return GAL_OK;
 }
 
+int gal_ode_batch_set_wall_limit(gal_ode_batch* h, double seconds) {
+  if (!h || !(seconds == seconds)) return GAL_E_INVALID;
+  if (seconds > 0.0 && !(h->wall_clock_hz > 0.0)) return GAL_E_HIP;
+  h->wall_limit_s = seconds > 0.0 ? seconds : 0.0;
+  return GAL_OK;
+}
+
 int gal_ode_batch_set_rhs_waves(gal_ode_batch* h, int waves) {
   if (!h || (waves != 1 && waves != 2)) return GAL_E_INVALID;
   h->rhs_waves = waves;
@@ -1026,6 +1056,9 @@ static int make_args(gal_ode_batch* h, int n, int ld, double* y, const double* n
   a.eps_rel = h->tol_rel;
   a.flags = h->flags;
   a.mode = (int)h->params[GAL_P_PHYSICS_MODE];
+  a.wall_ticks = 0;
+  if (h->wall_limit_s > 0.0)
+    a.wall_ticks = (unsigned long long)std::max(1.0, std::min(h->wall_limit_s * h->wall_clock_hz, 1.0e18));
   return GAL_OK;
 }
 
@@ -1266,6 +1299,7 @@ extern "C" int gal_ode_evolve_static_d(const double* params, int n_params, const
   a.eps_rel = tol_rel;
   a.flags = flags;
   a.mode = (int)params[GAL_P_PHYSICS_MODE];
+  a.wall_ticks = 0;
   // every lane vector in LDS, one workgroup per CU: no workspace beyond the scratch
   constexpr size_t lds_all = lds_bytes<64, true, true>();
   hipLaunchKernelGGL(reset_counter_kernel, dim3(1), dim3(64), 0, s, work);

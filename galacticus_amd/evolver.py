@@ -108,6 +108,12 @@ class NodeEvolverBatched:
         if rhs_waves is not None:
             _lib.check(self.L.gal_ode_batch_set_rhs_waves(self.handle, int(rhs_waves)), "gal_ode_batch_set_rhs_waves")
 
+    def set_wall_limit(self, seconds: float):
+        """Wall-time limit of each later evolve, in seconds from the start of
+        its launch (<= 0: none); nodes not finished by then return status
+        1025 (errorStatusXCPU) with their state as it came in."""
+        _lib.check(self.L.gal_ode_batch_set_wall_limit(self.handle, float(seconds)), "gal_ode_batch_set_wall_limit")
+
     def launch(self):
         """(nodes per workgroup, workgroups per CU, waves per group of nodes)."""
         bn, bb, bw = ctypes.c_int(), ctypes.c_int(), ctypes.c_int()

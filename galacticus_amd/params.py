@@ -179,6 +179,11 @@ FLAG_ENFORCE_NON_NEGATIVITY = 0x1
 FLAG_REUSE_STEP_SIZE = 0x2
 FLAG_SCALES_PROVIDED = 0x4
 FLAG_COST_ORDER = 0x8
+# per-node statuses (include/gal_ode.h; error/_module.F90 errorStatus*)
+STATUS_SUCCESS = 0
+STATUS_INTERRUPT = 9
+STATUS_UNDERFLOW = 15
+STATUS_XCPU = 1025
 
 
 def physics_params(**overrides) -> np.ndarray:
