@@ -97,6 +97,14 @@ __global__ void event_kernel(int n, const int* __restrict__ kind, const int* __r
     Y(m.to, b) += Y(m.from, a);
     Y(m.from, a) = 0.0;
   }
+  // the satellite's stripped gas joins the host's outflowed reservoir, as
+  // hot_halo/standard/_class.F90:532-597 moves it after each satellite
+  // evolve (here at the merger: mass and metals; its angular momentum,
+  // M_stripped J_host/M_host there, is not carried)
+  Y(GAL_V_HOT_OUTFLOWED_MASS, b) += Y(GAL_V_HOT_STRIPPED_MASS, a);
+  Y(GAL_V_HOT_STRIPPED_MASS, a) = 0.0;
+  Y(GAL_V_HOT_OUTFLOWED_ABUNDANCES, b) += Y(GAL_V_HOT_STRIPPED_ABUNDANCES, a);
+  Y(GAL_V_HOT_STRIPPED_ABUNDANCES, a) = 0.0;
   if (major) {
     const Move host_moves[] = {
         {GAL_V_DISK_MASS_GAS, GAL_V_SPHEROID_MASS_GAS},

@@ -1421,6 +1421,12 @@ static void merge_galaxy_simple_c(double *y, int ld, int sat, int host, double m
     Y(same[k], host) += Y(same[k], sat);
     Y(same[k], sat) = 0.0;
   }
+  /* stripped gas joins the host's outflowed reservoir (mass and metals;
+     hot_halo/standard/_class.F90:532-597 moves it after each evolve) */
+  Y(GAL_V_HOT_OUTFLOWED_MASS, host) += Y(GAL_V_HOT_STRIPPED_MASS, sat);
+  Y(GAL_V_HOT_STRIPPED_MASS, sat) = 0.0;
+  Y(GAL_V_HOT_OUTFLOWED_ABUNDANCES, host) += Y(GAL_V_HOT_STRIPPED_ABUNDANCES, sat);
+  Y(GAL_V_HOT_STRIPPED_ABUNDANCES, sat) = 0.0;
   if (major) {
     for (int k = 0; k < 6; k++) {
       Y(to_sph[k][1], host) += Y(to_sph[k][0], host);

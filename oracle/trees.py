@@ -304,7 +304,8 @@ def run_sequential(log, y0, h0, evolve_one, merge_galaxy):
 # State indices used by the mass movements.
 _V = dict(HOT_MASS=0, HOT_ABUNDANCES=1, HOT_ANGULAR_MOMENTUM=2, HOT_OUTFLOWED_MASS=3,
           HOT_OUTFLOWED_ANGULAR_MOMENTUM=4, HOT_OUTFLOWED_ABUNDANCES=5, HOT_UNACCRETED_MASS=6,
-          HOT_UNACCRETED_ABUNDANCES=7, HOT_ENERGY_RADIATED=11, DISK_MASS_STELLAR=12, DISK_MASS_STELLAR_FORMED=13,
+          HOT_UNACCRETED_ABUNDANCES=7, HOT_STRIPPED_MASS=9, HOT_STRIPPED_ABUNDANCES=10, HOT_ENERGY_RADIATED=11,
+          DISK_MASS_STELLAR=12, DISK_MASS_STELLAR_FORMED=13,
           DISK_ABUNDANCES_STELLAR=15, DISK_MASS_GAS=16, DISK_ABUNDANCES_GAS=17, DISK_ANGULAR_MOMENTUM=18,
           SPHEROID_MASS_STELLAR=19, SPHEROID_MASS_STELLAR_FORMED=20, SPHEROID_ABUNDANCES_STELLAR=21,
           SPHEROID_MASS_GAS=22, SPHEROID_ABUNDANCES_GAS=23, SPHEROID_ANGULAR_MOMENTUM=24, BH_MASS=25)
@@ -328,7 +329,10 @@ def merge_galaxy_simple(y, sat, host, major_ratio=0.25):
             "SPHEROID_MASS_STELLAR_FORMED", "SPHEROID_ANGULAR_MOMENTUM", "HOT_MASS", "HOT_ABUNDANCES",
             "HOT_ANGULAR_MOMENTUM", "HOT_OUTFLOWED_MASS", "HOT_OUTFLOWED_ANGULAR_MOMENTUM", "HOT_OUTFLOWED_ABUNDANCES",
             "HOT_UNACCRETED_MASS", "HOT_UNACCRETED_ABUNDANCES", "HOT_ENERGY_RADIATED", "BH_MASS"]
-    for a, b in to_sph + [(k, k) for k in same]:
+    # the satellite's stripped gas joins the host's outflowed reservoir (mass
+    # and metals; hot_halo/standard/_class.F90:532-597)
+    stripped = [("HOT_STRIPPED_MASS", "HOT_OUTFLOWED_MASS"), ("HOT_STRIPPED_ABUNDANCES", "HOT_OUTFLOWED_ABUNDANCES")]
+    for a, b in to_sph + [(k, k) for k in same] + stripped:
         y[V[b], host] += y[V[a], sat]
         y[V[a], sat] = 0.0
     if major:

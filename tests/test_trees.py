@@ -312,7 +312,8 @@ def test_merge_galaxy_simple_hand_cases(major):
     (satellites/merging/mass_movements/simple.F90:208-290), by hand: the
     satellite's cold gas and stars (and their metals, formed-star tallies and
     angular momentum) land in the host spheroid, its circumgalactic
-    reservoirs and black hole join the host's; in a major merger (smaller
+    reservoirs and black hole join the host's, its stripped gas the host's
+    outflowed reservoir; in a major merger (smaller
     galaxy at least 0.25 of the larger) the host disk moves to the spheroid
     too; the satellite is left empty."""
     V = wl.V
@@ -320,6 +321,8 @@ def test_merge_galaxy_simple_hand_cases(major):
     sat, host = 0, 1
     y[V["DISK_MASS_GAS"], sat], y[V["DISK_MASS_STELLAR"], sat], y[V["SPHEROID_MASS_STELLAR"], sat] = 1.0, 2.0, 0.5
     y[V["DISK_ABUNDANCES_GAS"], sat], y[V["HOT_MASS"], sat], y[V["BH_MASS"], sat] = 0.01, 3.0, 1e-3
+    y[V["HOT_STRIPPED_MASS"], sat], y[V["HOT_STRIPPED_ABUNDANCES"], sat] = 0.7, 0.007
+    y[V["HOT_OUTFLOWED_MASS"], host] = 1.5
     y[V["DISK_MASS_GAS"], host], y[V["DISK_MASS_STELLAR"], host] = 4.0, (6.0 if major else 40.0)
     y[V["SPHEROID_MASS_GAS"], host], y[V["HOT_MASS"], host], y[V["BH_MASS"], host] = 0.2, 10.0, 2e-3
     before = y.copy()
@@ -330,6 +333,9 @@ def test_merge_galaxy_simple_hand_cases(major):
     assert np.all(y[:, sat][[V[k] for k in ("DISK_MASS_GAS", "DISK_MASS_STELLAR", "SPHEROID_MASS_STELLAR",
                                              "DISK_ABUNDANCES_GAS", "HOT_MASS", "BH_MASS")]] == 0.0)
     assert y[V["HOT_MASS"], host] == 13.0 and y[V["BH_MASS"], host] == 3e-3
+    # the satellite's stripped gas joins the host's outflowed reservoir
+    assert y[V["HOT_OUTFLOWED_MASS"], host] == 1.5 + 0.7 and y[V["HOT_OUTFLOWED_ABUNDANCES"], host] == 0.007
+    assert y[V["HOT_STRIPPED_MASS"], sat] == 0.0 and y[V["HOT_STRIPPED_ABUNDANCES"], sat] == 0.0
     if major:
         assert y[V["SPHEROID_MASS_GAS"], host] == 0.2 + 1.0 + 4.0
         assert y[V["SPHEROID_MASS_STELLAR"], host] == 0.5 + 2.0 + 6.0
@@ -340,6 +346,6 @@ def test_merge_galaxy_simple_hand_cases(major):
         assert y[V["DISK_MASS_GAS"], host] == 4.0 and y[V["DISK_MASS_STELLAR"], host] == 40.0
     assert y[V["SPHEROID_ABUNDANCES_GAS"], host] == 0.01
     # total baryons conserved
-    mass = [V[k] for k in ("HOT_MASS", "DISK_MASS_GAS", "DISK_MASS_STELLAR", "SPHEROID_MASS_GAS",
-                           "SPHEROID_MASS_STELLAR", "BH_MASS")]
+    mass = [V[k] for k in ("HOT_MASS", "HOT_OUTFLOWED_MASS", "HOT_STRIPPED_MASS", "DISK_MASS_GAS",
+                           "DISK_MASS_STELLAR", "SPHEROID_MASS_GAS", "SPHEROID_MASS_STELLAR", "BH_MASS")]
     assert np.isclose(y[mass].sum(), before[mass].sum(), rtol=1e-15)
