@@ -354,6 +354,44 @@ static int tree_evolve(gal_ode_batch* h, const gal_schedule* sched, int ld, doub
   return rc;
 }
 
+extern "C" int gal_node_events_h(int n_events, const int* kind, const int* node, const int* target, int n_nodes,
+                                 double* y, double* h_step, double major_ratio) {
+  if (n_events < 0 || n_nodes < 1 || (n_events > 0 && (!kind || !node || !target || !y || !h_step)))
+    return GAL_E_INVALID;
+  if (n_events == 0) return GAL_OK;
+  // events run in parallel on the device: each must name valid, distinct nodes
+  std::vector<char> used((size_t)n_nodes, 0);
+  for (int i = 0; i < n_events; ++i) {
+    if (kind[i] != GAL_OP_PROMOTE && kind[i] != GAL_OP_MERGE_GALAXY) return GAL_E_INVALID;
+    const int a = node[i], b = target[i];
+    if (a < 0 || a >= n_nodes || b < 0 || b >= n_nodes || a == b || used[a] || used[b]) return GAL_E_INVALID;
+    used[a] = used[b] = 1;
+  }
+  int *d_kind = nullptr, *d_a = nullptr, *d_b = nullptr;
+  double *d_y = nullptr, *d_h = nullptr;
+  const size_t ny = sizeof(double) * GAL_NVAR * (size_t)n_nodes, nh = sizeof(double) * (size_t)n_nodes,
+               ne = sizeof(int) * (size_t)n_events;
+  int rc = GAL_OK;
+  if (hipMalloc(&d_kind, ne) != hipSuccess || hipMalloc(&d_a, ne) != hipSuccess || hipMalloc(&d_b, ne) != hipSuccess ||
+      hipMalloc(&d_y, ny) != hipSuccess || hipMalloc(&d_h, nh) != hipSuccess) {
+    rc = GAL_E_NOMEM;
+  } else if (hipMemcpy(d_kind, kind, ne, hipMemcpyHostToDevice) != hipSuccess ||
+             hipMemcpy(d_a, node, ne, hipMemcpyHostToDevice) != hipSuccess ||
+             hipMemcpy(d_b, target, ne, hipMemcpyHostToDevice) != hipSuccess ||
+             hipMemcpy(d_y, y, ny, hipMemcpyHostToDevice) != hipSuccess ||
+             hipMemcpy(d_h, h_step, nh, hipMemcpyHostToDevice) != hipSuccess) {
+    rc = GAL_E_HIP;
+  } else {
+    hipLaunchKernelGGL(event_kernel, dim3((n_events + 255) / 256), dim3(256), 0, 0, n_events, d_kind, d_a, d_b,
+                       n_nodes, d_y, d_h, major_ratio);
+    if (hipGetLastError() != hipSuccess || hipMemcpy(y, d_y, ny, hipMemcpyDeviceToHost) != hipSuccess ||
+        hipMemcpy(h_step, d_h, nh, hipMemcpyDeviceToHost) != hipSuccess)
+      rc = GAL_E_HIP;
+  }
+  hipFree(d_kind); hipFree(d_a); hipFree(d_b); hipFree(d_y); hipFree(d_h);
+  return rc;
+}
+
 extern "C" int gal_tree_evolve_d(gal_ode_batch* h, const gal_schedule* sched, int ld, double* y, double* scale,
                                  int* status, double major_ratio, void* stream) {
   return tree_evolve(h, sched, ld, y, scale, status, major_ratio, stream, nullptr);

@@ -91,6 +91,24 @@ def walk_params(time_end, timestep_relative=0.1, timestep_absolute=1.0, timestep
                       orbit_circularity, spin_parameter)
 
 
+def node_events(kind, node, target, y, h_step, major_ratio: float = 0.25):
+    """Apply promote/merge events (GAL_OP_PROMOTE, GAL_OP_MERGE_GALAXY) to
+    host arrays y [30, n] and h_step [n] in place on the GPU; every node may
+    appear in at most one event."""
+    L = _bind(_lib.load())
+    kind = np.ascontiguousarray(kind, dtype=np.int32)
+    node = np.ascontiguousarray(node, dtype=np.int32)
+    target = np.ascontiguousarray(target, dtype=np.int32)
+    if not (y.flags.c_contiguous and y.dtype == np.float64 and h_step.flags.c_contiguous and h_step.dtype == np.float64):
+        raise ValueError("y and h_step must be C-contiguous float64 arrays")
+    if y.shape != (gp.NVAR, h_step.shape[0]) or not (len(kind) == len(node) == len(target)):
+        raise ValueError("inconsistent event or state shapes")
+    i32p, dp = ctypes.POINTER(ctypes.c_int32), ctypes.POINTER(ctypes.c_double)
+    _lib.check(L.gal_node_events_h(len(kind), kind.ctypes.data_as(i32p), node.ctypes.data_as(i32p),
+                                   target.ctypes.data_as(i32p), h_step.shape[0], y.ctypes.data_as(dp),
+                                   h_step.ctypes.data_as(dp), float(major_ratio)), "gal_node_events_h")
+
+
 def _bind(L):
     if getattr(L, "_trees_bound", False):
         return L
@@ -111,6 +129,7 @@ def _bind(L):
     L.gal_forest_initial_state.argtypes = [vp, ctypes.POINTER(Cosmology), ctypes.POINTER(WalkParams),
                                            ctypes.c_double, ctypes.c_double, c_int, dp]
     L.gal_tree_evolve_d.argtypes = [vp, vp, c_int, vp, vp, vp, ctypes.c_double, vp]
+    L.gal_node_events_h.argtypes = [c_int, ip, ip, ip, c_int, dp, dp, ctypes.c_double]
     L.gal_tree_evolve_surrogate_d.argtypes = [vp, vp, vp, ctypes.POINTER(_lib.Gate), vp, c_int, vp, vp, vp,
                                               ctypes.c_double, ctypes.POINTER(ctypes.c_int64), vp]
     L.gal_tree_evolve_record_d.argtypes = [vp, vp, c_int, vp, vp, vp, ctypes.c_double, vp, vp, vp, vp, vp,
@@ -118,7 +137,8 @@ def _bind(L):
     for name in ("gal_forest_build_pch", "gal_forest_from_arrays", "gal_forest_size", "gal_forest_arrays",
                  "gal_forest_destroy", "gal_cosmology_sigma", "gal_cosmology_delta_critical", "gal_schedule_build",
                  "gal_schedule_size", "gal_schedule_ops", "gal_schedule_destroy", "gal_forest_initial_state",
-                 "gal_tree_evolve_d", "gal_tree_evolve_surrogate_d", "gal_tree_evolve_record_d"):
+                 "gal_tree_evolve_d", "gal_tree_evolve_surrogate_d", "gal_tree_evolve_record_d",
+                 "gal_node_events_h"):
         getattr(L, name).restype = c_int
     L._trees_bound = True
     return L

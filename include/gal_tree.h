@@ -119,6 +119,19 @@ int gal_forest_initial_state(const gal_forest* f, const gal_cosmology* cosmo, co
 int gal_tree_evolve_d(gal_ode_batch* h, const gal_schedule* s, int ld, double* y, double* scale, int* status,
                       double major_ratio, void* stream);
 
+/* Apply node events to host arrays (copied to the device and back): each
+   event is GAL_OP_PROMOTE (node's galaxy moves into its parent target:
+   mergerTreeNodeEvolverStandard%promote, node_evolver/standard.F90:1241-1356)
+   or GAL_OP_MERGE_GALAXY (satellite node merges into host target:
+   mergerTreeNodeEvolverStandard%merge through mergerMassMovementsSimple,
+   satellites/merging/mass_movements/simple.F90:208-290, the satellite's
+   stripped gas joining the host's outflowed reservoir).  y: SoA
+   [GAL_NVAR*n_nodes]; h_step [n_nodes] (a promotion hands the node's step
+   size to its parent).  Events run in parallel, so every node may appear in
+   at most one of them (GAL_E_INVALID otherwise). */
+int gal_node_events_h(int n_events, const int* kind, const int* node, const int* target, int n_nodes, double* y,
+                      double* h_step, double major_ratio);
+
 /* The same with the learned surrogate in front of the integrator
    (include/gal_surrogate.h): in every wave the evolve operations of halos
    that are not satellites go through gal_surrogate_evolve_d (network, gate,
