@@ -277,7 +277,7 @@ def dataclasses_asdict(x):
     return dataclasses.asdict(x)
 
 
-PIPELINE_SHAPES = ((64, 1), (64, 2), (64, 4))
+PIPELINE_SHAPES = ((64, 1, 1), (64, 2, 1), (64, 4, 1), (64, 1, 2), (64, 2, 2))
 
 
 def pipelined_surrogate(ev, mlp, norm, gate, test, h0, launch, physics, table, tol_abs, tol_rel, args):
@@ -343,7 +343,8 @@ def pipelined_surrogate(ev, mlp, norm, gate, test, h0, launch, physics, table, t
         out[f"{name}_s_per_batch"] = float(np.median([timed_pass(used) for _ in range(3)]))
         out[f"{name}_launch"] = list(best)
         out[f"{name}_streams"] = used
-        out[f"{name}_s_per_batch_by_config"] = {f"{u} streams, {a},{b}": v for (u, (a, b)), v in per_config.items()}
+        out[f"{name}_s_per_batch_by_config"] = {f"{u} streams, {a},{b},{c}": v
+                                                  for (u, (a, b, c)), v in per_config.items()}
         if name == "surrogate":
             out["rejected_fraction_last_batch"] = int((accepted[0] == 0).sum()) / n
         out[f"{name}_statuses_ok"] = all(bool((st == 0).all()) for st in status)
