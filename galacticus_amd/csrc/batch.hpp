@@ -8,6 +8,7 @@
 
 #include "../../include/gal_ode.h"
 #include "physics.hpp"
+#include "events.hpp"
 
 struct gal_ode_batch {
   int n_max = 0;
@@ -61,3 +62,12 @@ int gal_ode_evolve_subset(gal_ode_batch* h, int n, int ld, double* y, const doub
                           const double* t_end, double* h_step, double* scale, int* status, int64_t* counters,
                           const int* subset, int n_subset, hipStream_t s,
                           const int* n_subset_dev = nullptr);
+
+// Run a forest's operation graph (events.hpp) in one persistent launch:
+// lanes claim ready operations, evolve intervals in place on the slot-indexed
+// state, apply events inline and release the operations that follow.  Every
+// operation of one slot runs in graph order, so the state equals that of the
+// per-wave executor bit for bit.  *graph_error (device) is set if lanes gave
+// up waiting on a graph that stopped making progress.
+int gal_ode_evolve_graph(gal_ode_batch* h, int ld, double* y, double* h_step, double* scale, int* status,
+                         const gal::OpGraph& g, int* graph_error, hipStream_t s);

@@ -48,6 +48,7 @@
 #include "physics.hpp"
 #include "integrator.hpp"
 #include "batch.hpp"
+#include "events.hpp"
 
 #include <hipcub/device/device_radix_sort.hpp>
 
@@ -85,6 +86,14 @@ struct EvolveArgs {
   // realtime counter (0: none).  An attempt that would start after it
   // returns the node with errorStatusXCPU and its state untouched.
   unsigned long long wall_ticks;
+  // Operation-graph mode (kernel parameter GRAPH): lanes claim entries of
+  // the graph's ready list (a.work counts the claims) instead of nodes.  A
+  // lane whose claimed entry stays unwritten while no operation is appended
+  // for stall_ticks gives up, sets *graph_error and leaves, so that a
+  // malformed graph ends the launch instead of holding it.
+  OpGraph g;
+  int* graph_error;
+  unsigned long long stall_ticks;
 };
 
 // Claim one node index per requesting lane with a single atomic per wave.
@@ -193,7 +202,7 @@ __device__ __forceinline__ void rhs_coprocessor(const double* P, const double* _
   }
 }
 
-template <int BLOCK, bool YLDS, bool KLDS, int WAVES = 1>
+template <int BLOCK, bool YLDS, bool KLDS, int WAVES = 1, bool GRAPH = false>
 __global__ __launch_bounds__(BLOCK * WAVES, 1) void evolve_kernel(EvolveArgs a) {
   extern __shared__ double lds_k[];
   __shared__ int xchg_cmd;
@@ -307,6 +316,9 @@ __global__ __launch_bounds__(BLOCK * WAVES, 1) void evolve_kernel(EvolveArgs a) 
   };
   NodePar np{};
   int node = -1;
+  int op = -1, pend = -1;                  // graph mode: operation in hand, claimed ready-list entry
+  unsigned int wait_tail = ~0u;            // graph mode: appends seen while waiting on the entry
+  unsigned long long wait_since = 0ull;
   bool active = false, want = true, eval_pending = false, post_pending = false, final_step = false;
   double t = 0.0, t0 = 0.0, t1 = 0.0, h = 0.0, h0 = 0.0, h_node = 0.0, sign = 1.0;
   int trial = 0;
@@ -323,69 +335,124 @@ __global__ __launch_bounds__(BLOCK * WAVES, 1) void evolve_kernel(EvolveArgs a) 
     bool do_rhs = false;
     if (phase == 5) {
       // ---- claim nodes for lanes that need one (convergent point) ------
-      while (__any(want)) {
-        const int idx = wave_claim(want, a.work, n_claim, a.order);
-        if (want) {
-          if (idx < 0) {
-            want = false;
-            active = false;
-          } else {
-            node = idx;
-            np = load_node_par(a.np, a.ld, node);
-            if constexpr (WAVES == 2) {
-              XC[(kNV + 1) * BLOCK] = np.t_ref;
-              XC[(kNV + 2) * BLOCK] = np.m_ref;
-              XC[(kNV + 3) * BLOCK] = np.m_rate;
-              XC[(kNV + 4) * BLOCK] = np.spin_rate;
-              XC[(kNV + 5) * BLOCK] = np.ram_force;
-              XC[(kNV + 6) * BLOCK] = np.v_host;
-              XC[(kNV + 7) * BLOCK] = np.satellite ? 1.0 : 0.0;
+      bool again = true;
+      while (again) {
+        bool begun = false;   // this lane takes a node's interval
+        if constexpr (GRAPH) {
+          // claim an entry of the ready list, then wait for it to be written;
+          // events are applied on the spot and their successors released
+          const bool need = want && pend < 0;
+          if (__any(need)) {
+            const int j = wave_claim(need, a.work, a.g.n, nullptr);
+            if (need) {
+              if (j < 0) {
+                want = false;
+                active = false;
+              } else {
+                pend = j;
+                wait_tail = ~0u;
+              }
             }
-            t0 = a.t_start[node];
-            t1 = a.t_end[node];
-            h_node = a.h_step[node];
-#pragma unroll
-            for (int v = 0; v < kNV; ++v) ytmp[v] = a.y[v * (size_t)a.ld + node];
-            if (scales_given) {
-#pragma unroll
-              for (int v = 0; v < kNV; ++v) kout[v] = a.scale[v * (size_t)a.ld + node];
-            } else if (mode == GAL_PHYSICS_TEST) {
-#pragma unroll
-              for (int v = 0; v < kNV; ++v) kout[v] = 1.0;
+          }
+          if (want && pend >= 0) {
+            const int k = graph_ready_entry(a.g, pend);
+            if (k >= 0) {
+              pend = -1;
+              const int kind = a.g.kind[k];
+              if (kind != GAL_OP_EVOLVE) {
+                apply_event(kind, (size_t)a.g.a[k], (size_t)a.g.b[k], a.ld, a.y, a.h_step, a.g.major_ratio);
+                graph_release(a.g, k);
+              } else {
+                op = k;
+                node = a.g.a[k];
+                np = load_node_par(a.g.np, a.g.n, k);
+                t0 = a.g.t0[k];
+                t1 = a.g.t1[k];
+                begun = true;
+              }
             } else {
-              scales(P, a.D, np, t0, ytmp, kout);
+              const unsigned int tl = __hip_atomic_load(a.g.tail, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+              const unsigned long long now = wall_clock64();
+              if (tl != wait_tail) {
+                wait_tail = tl;
+                wait_since = now;
+              } else if (now - wait_since > a.stall_ticks) {
+                __hip_atomic_store(a.graph_error, 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+                want = false;
+                pend = -1;
+              }
             }
-            if (!scales_given && a.scale) {
-#pragma unroll
-              for (int v = 0; v < kNV; ++v) a.scale[v * (size_t)a.ld + node] = kout[v];
-            }
-#pragma unroll
-            for (int v = 0; v < kNV; ++v) {
-              Y(v) = ytmp[v];
-              SC(v) = kout[v];
-            }
-            n_attempt = n_accept = n_rhs = 0;
-            trial = 0;
-            if (t0 == t1) {
-              // nothing to integrate (standard.F90:580): success, timeStep -1
-              a.h_step[node] = -1.0;
-              a.status[node] = GAL_STATUS_SUCCESS;
-              write_counters(a, node, 0, 0, 0);
-            } else {
+          }
+        } else {
+          const int idx = wave_claim(want, a.work, n_claim, a.order);
+          if (want) {
+            if (idx < 0) {
               want = false;
-              active = true;
-              // odeSolverSolve initial step (solver.F90:537-555)
-              const double step_size = reuse ? h_node : -1.0;
-              double xs = t1 - t0;
-              if (step_size > 0.0) xs = fmin(step_size, xs);
-              h = xs;
-              h0 = h;
-              sign = h > 0.0 ? 1.0 : -1.0;
-              t = t0;
-              eval_pending = true;
+              active = false;
+            } else {
+              node = idx;
+              np = load_node_par(a.np, a.ld, node);
+              t0 = a.t_start[node];
+              t1 = a.t_end[node];
+              begun = true;
             }
           }
         }
+        if (begun) {
+          if constexpr (WAVES == 2) {
+            XC[(kNV + 1) * BLOCK] = np.t_ref;
+            XC[(kNV + 2) * BLOCK] = np.m_ref;
+            XC[(kNV + 3) * BLOCK] = np.m_rate;
+            XC[(kNV + 4) * BLOCK] = np.spin_rate;
+            XC[(kNV + 5) * BLOCK] = np.ram_force;
+            XC[(kNV + 6) * BLOCK] = np.v_host;
+            XC[(kNV + 7) * BLOCK] = np.satellite ? 1.0 : 0.0;
+          }
+          h_node = a.h_step[node];
+#pragma unroll
+          for (int v = 0; v < kNV; ++v) ytmp[v] = a.y[v * (size_t)a.ld + node];
+          if (scales_given) {
+#pragma unroll
+            for (int v = 0; v < kNV; ++v) kout[v] = a.scale[v * (size_t)a.ld + node];
+          } else if (mode == GAL_PHYSICS_TEST) {
+#pragma unroll
+            for (int v = 0; v < kNV; ++v) kout[v] = 1.0;
+          } else {
+            scales(P, a.D, np, t0, ytmp, kout);
+          }
+          if (!scales_given && a.scale) {
+#pragma unroll
+            for (int v = 0; v < kNV; ++v) a.scale[v * (size_t)a.ld + node] = kout[v];
+          }
+#pragma unroll
+          for (int v = 0; v < kNV; ++v) {
+            Y(v) = ytmp[v];
+            SC(v) = kout[v];
+          }
+          n_attempt = n_accept = n_rhs = 0;
+          trial = 0;
+          if (t0 == t1) {
+            // nothing to integrate (standard.F90:580): success, timeStep -1
+            a.h_step[node] = -1.0;
+            a.status[node] = GAL_STATUS_SUCCESS;
+            write_counters(a, node, 0, 0, 0);
+            if constexpr (GRAPH) graph_release(a.g, op);
+          } else {
+            want = false;
+            active = true;
+            // odeSolverSolve initial step (solver.F90:537-555)
+            const double step_size = reuse ? h_node : -1.0;
+            double xs = t1 - t0;
+            if (step_size > 0.0) xs = fmin(step_size, xs);
+            h = xs;
+            h0 = h;
+            sign = h > 0.0 ? 1.0 : -1.0;
+            t = t0;
+            eval_pending = true;
+          }
+        }
+        if constexpr (GRAPH) again = __any(want && pend < 0);
+        else again = __any(want);
       }
       if (eval_pending) {
         do_rhs = true;
@@ -403,6 +470,7 @@ __global__ __launch_bounds__(BLOCK * WAVES, 1) void evolve_kernel(EvolveArgs a) 
           if (a.wall_ticks && wall_clock64() - t_launch > a.wall_ticks) {
             a.status[node] = GAL_STATUS_XCPU;
             write_counters(a, node, n_attempt, n_accept, n_rhs);
+            if constexpr (GRAPH) graph_release(a.g, op);
             active = false;
             want = true;
             do_rhs = false;
@@ -516,6 +584,13 @@ __global__ __launch_bounds__(BLOCK * WAVES, 1) void evolve_kernel(EvolveArgs a) 
         }
       }
       if (!__any(active)) {
+        if constexpr (GRAPH) {
+          // lanes wait on ready-list entries: poll again after a pause
+          if (__any(want)) {
+            __builtin_amdgcn_s_sleep(8);
+            continue;
+          }
+        }
         if constexpr (WAVES == 2) {
           if (threadIdx.x == 0) xchg_cmd = kCmdQuit;
           __syncthreads();
@@ -573,6 +648,7 @@ __global__ __launch_bounds__(BLOCK * WAVES, 1) void evolve_kernel(EvolveArgs a) 
         a.h_step[node] = h;
         a.status[node] = GAL_STATUS_SUCCESS;
         write_counters(a, node, n_attempt, n_accept, n_rhs);
+        if constexpr (GRAPH) graph_release(a.g, op);
         active = false;
         want = true;
       }
@@ -597,6 +673,7 @@ __global__ __launch_bounds__(BLOCK * WAVES, 1) void evolve_kernel(EvolveArgs a) 
         a.h_step[node] = h_node / (double)(1 << trial);
         a.status[node] = GAL_STATUS_SUCCESS;
         write_counters(a, node, n_attempt, n_accept, n_rhs);
+        if constexpr (GRAPH) graph_release(a.g, op);
         active = false;
         want = true;
       } else if (++trial < 8) {
@@ -613,6 +690,7 @@ __global__ __launch_bounds__(BLOCK * WAVES, 1) void evolve_kernel(EvolveArgs a) 
         // all trials failed: state restored, errorStatusUnderflow (standard.F90:715-718)
         a.status[node] = GAL_STATUS_UNDERFLOW;
         write_counters(a, node, n_attempt, n_accept, n_rhs);
+        if constexpr (GRAPH) graph_release(a.g, op);
         active = false;
         want = true;
       }
@@ -708,6 +786,25 @@ template <int BLOCK, bool YLDS, bool KLDS, int WAVES = 1>
 constexpr size_t lds_bytes() {
   return lds_vector_bytes<BLOCK, YLDS, KLDS>() + (WAVES == 2 ? sizeof(double) * kXchgVectors * BLOCK : 0);
 }
+
+// The dynamic LDS of every launch shape of the evolve kernel, in node mode
+// and in operation-graph mode.
+template <bool GRAPH>
+hipError_t set_lds_attributes() {
+  const struct { const void* f; size_t bytes; } shapes[] = {
+      {(const void*)evolve_kernel<64, true, true, 2, GRAPH>, lds_bytes<64, true, true, 2>()},
+      {(const void*)evolve_kernel<64, true, false, 2, GRAPH>, lds_bytes<64, true, false, 2>()},
+      {(const void*)evolve_kernel<64, true, true, 1, GRAPH>, lds_bytes<64, true, true>()},
+      {(const void*)evolve_kernel<64, false, true, 1, GRAPH>, lds_bytes<64, false, true>()},
+      {(const void*)evolve_kernel<64, true, false, 1, GRAPH>, lds_bytes<64, true, false>()},
+  };
+  for (const auto& sh : shapes) {
+    const hipError_t e = hipFuncSetAttribute(sh.f, hipFuncAttributeMaxDynamicSharedMemorySize, (int)sh.bytes);
+    if (e != hipSuccess) return e;
+  }
+  return hipSuccess;
+}
+
 
 }  // namespace
 
@@ -852,18 +949,7 @@ int gal_ode_batch_create(int n_max, double tol_abs, double tol_rel, int flags, g
     static_assert(4 * lds_bytes<64, true, false>() <= 163840, "four workgroups per CU must fit 160 KiB");
     static_assert(lds_bytes<64, true, true, 2>() + 16 <= 163840, "two-wave groups with LDS stages must fit 160 KiB");
     static_assert(2 * (lds_bytes<64, true, false, 2>() + 16) <= 163840, "two two-wave groups per CU must fit");
-    if (hipFuncSetAttribute((const void*)evolve_kernel<64, true, true, 2>,
-                            hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds_bytes<64, true, true, 2>()) !=
-            hipSuccess ||
-        hipFuncSetAttribute((const void*)evolve_kernel<64, true, false, 2>,
-                            hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds_bytes<64, true, false, 2>()) !=
-            hipSuccess ||
-        hipFuncSetAttribute((const void*)evolve_kernel<64, true, true>, hipFuncAttributeMaxDynamicSharedMemorySize,
-                            (int)lds_bytes<64, true, true>()) != hipSuccess ||
-        hipFuncSetAttribute((const void*)evolve_kernel<64, false, true>, hipFuncAttributeMaxDynamicSharedMemorySize,
-                            (int)lds_bytes<64, false, true>()) != hipSuccess ||
-        hipFuncSetAttribute((const void*)evolve_kernel<64, true, false>, hipFuncAttributeMaxDynamicSharedMemorySize,
-                            (int)lds_bytes<64, true, false>()) != hipSuccess) {
+    if (set_lds_attributes<false>() != hipSuccess || set_lds_attributes<true>() != hipSuccess) {
       (void)hipGetLastError();
       gal_ode_batch_destroy(h);
       return GAL_E_HIP;
@@ -984,6 +1070,8 @@ int gal_ode_batch_get_launch(gal_ode_batch* h, int* nodes_per_block, int* blocks
   return GAL_OK;
 }
 
+extern "C++" {
+template <bool GRAPH = false>
 static int launch_evolve(gal_ode_batch* h, const EvolveArgs& args, hipStream_t s) {
   const int blocks = std::max(1, h->n_cus * h->blocks_per_cu);
   if (h->rhs_waves == 2 && h->blocks_per_cu > 2) return GAL_E_INVALID;
@@ -993,13 +1081,13 @@ static int launch_evolve(gal_ode_batch* h, const EvolveArgs& args, hipStream_t s
     // derivatives in LDS, or two with them in the HBM workspace
     if (h->blocks_per_cu == 1) {
       constexpr size_t lds = lds_bytes<64, true, true, 2>();
-      hipLaunchKernelGGL((evolve_kernel<64, true, true, 2>), dim3(blocks), dim3(128), lds, s, args);
+      hipLaunchKernelGGL((evolve_kernel<64, true, true, 2, GRAPH>), dim3(blocks), dim3(128), lds, s, args);
     } else {
       EvolveArgs b = args;
       b.kwork = h->d_kwork;
       if (blocks * 64 > h->kwork_lanes) return GAL_E_INVALID;
       constexpr size_t lds = lds_bytes<64, true, false, 2>();
-      hipLaunchKernelGGL((evolve_kernel<64, true, false, 2>), dim3(blocks), dim3(128), lds, s, b);
+      hipLaunchKernelGGL((evolve_kernel<64, true, false, 2, GRAPH>), dim3(blocks), dim3(128), lds, s, b);
     }
     HIP_TRY(hipGetLastError());
     return GAL_OK;
@@ -1007,22 +1095,23 @@ static int launch_evolve(gal_ode_batch* h, const EvolveArgs& args, hipStream_t s
   constexpr size_t lds_all = lds_bytes<64, true, true>(), lds_k = lds_bytes<64, false, true>(),
                    lds_y = lds_bytes<64, true, false>();
   if (h->blocks_per_cu == 1) {
-    hipLaunchKernelGGL((evolve_kernel<64, true, true>), dim3(blocks), dim3(64), lds_all, s, args);
+    hipLaunchKernelGGL((evolve_kernel<64, true, true, 1, GRAPH>), dim3(blocks), dim3(64), lds_all, s, args);
   } else if (h->blocks_per_cu == 2) {
     EvolveArgs b = args;
     b.ywork = h->d_ywork;
     b.ystride = blocks * 64;
     if (b.ystride > h->ywork_lanes) return GAL_E_INVALID;
-    hipLaunchKernelGGL((evolve_kernel<64, false, true>), dim3(blocks), dim3(64), lds_k, s, b);
+    hipLaunchKernelGGL((evolve_kernel<64, false, true, 1, GRAPH>), dim3(blocks), dim3(64), lds_k, s, b);
   } else {
     EvolveArgs b = args;
     b.kwork = h->d_kwork;
     if (blocks * 64 > h->kwork_lanes) return GAL_E_INVALID;
-    hipLaunchKernelGGL((evolve_kernel<64, true, false>), dim3(blocks), dim3(64), lds_y, s, b);
+    hipLaunchKernelGGL((evolve_kernel<64, true, false, 1, GRAPH>), dim3(blocks), dim3(64), lds_y, s, b);
   }
   HIP_TRY(hipGetLastError());
   return GAL_OK;
 }
+}  // extern "C++"
 
 static int make_args(gal_ode_batch* h, int n, int ld, double* y, const double* node_par, const double* t_start,
                      const double* t_end, double* h_step, double* scale, int* status, int64_t* counters,
@@ -1332,4 +1421,25 @@ int gal_ode_evolve_subset(gal_ode_batch* h, int n, int ld, double* y, const doub
     a.order = h->d_order;
   }
   return launch_evolve(h, a, s);
+}
+
+int gal_ode_evolve_graph(gal_ode_batch* h, int ld, double* y, double* h_step, double* scale, int* status,
+                         const gal::OpGraph& g, int* graph_error, hipStream_t s) {
+  if (!h || ld < 1 || ld > h->n_max || g.n < 0 || !graph_error) return GAL_E_INVALID;
+  if (g.n == 0) return GAL_OK;
+  if (!g.kind || !g.a || !g.b || !g.next_a || !g.next_b || !g.t0 || !g.t1 || !g.np || !g.npred || !g.ready ||
+      !g.tail)
+    return GAL_E_INVALID;
+  EvolveArgs a;
+  // the per-node arrays of the node mode are unused: intervals and halo
+  // parameters come with each operation
+  int rc = make_args(h, ld, ld, y, g.np, g.t0, g.t1, h_step, scale, status, nullptr, a);
+  if (rc != GAL_OK) return rc;
+  a.g = g;
+  a.graph_error = graph_error;
+  // a claimed entry may wait for the whole remaining critical path, but
+  // not for a minute in which no operation anywhere becomes ready
+  const double hz = h->wall_clock_hz > 0.0 ? h->wall_clock_hz : 1.0e8;
+  a.stall_ticks = (unsigned long long)(60.0 * hz);
+  return launch_evolve<true>(h, a, s);
 }

@@ -16,6 +16,7 @@
 //      them; the satellite's hot-halo reservoirs and black hole join the host's).
 #include <hip/hip_runtime.h>
 #include <algorithm>
+#include <cstdio>
 #include <cstdint>
 #include <vector>
 
@@ -24,6 +25,7 @@
 #include "../../include/gal_surrogate.h"
 #include "physics.hpp"
 #include "batch.hpp"
+#include "events.hpp"
 
 using namespace gal;
 
@@ -52,73 +54,7 @@ __global__ void event_kernel(int n, const int* __restrict__ kind, const int* __r
                              double major_ratio) {
   const int i = blockIdx.x * blockDim.x + threadIdx.x;
   if (i >= n) return;
-  const size_t a = (size_t)from[i], b = (size_t)to[i];
-  auto Y = [&](int v, size_t s) -> double& { return y[(size_t)v * ld + s]; };
-  if (kind[i] == GAL_OP_PROMOTE) {
-#pragma unroll
-    for (int v = 0; v < kNV; ++v) Y(v, b) = Y(v, a);
-    h_step[b] = h_step[a];
-    return;
-  }
-  // GAL_OP_MERGE_GALAXY: satellite a into host b
-  const double m_sat = Y(GAL_V_DISK_MASS_GAS, a) + Y(GAL_V_DISK_MASS_STELLAR, a) + Y(GAL_V_SPHEROID_MASS_GAS, a) +
-                       Y(GAL_V_SPHEROID_MASS_STELLAR, a);
-  const double m_host = Y(GAL_V_DISK_MASS_GAS, b) + Y(GAL_V_DISK_MASS_STELLAR, b) + Y(GAL_V_SPHEROID_MASS_GAS, b) +
-                        Y(GAL_V_SPHEROID_MASS_STELLAR, b);
-  const bool major = m_sat > 0.0 && m_host > 0.0 && fmin(m_sat, m_host) >= major_ratio * fmax(m_sat, m_host);
-  struct Move { int from, to; };
-  // satellite disk and spheroid -> host spheroid
-  const Move sat_moves[] = {
-      {GAL_V_DISK_MASS_GAS, GAL_V_SPHEROID_MASS_GAS},
-      {GAL_V_DISK_ABUNDANCES_GAS, GAL_V_SPHEROID_ABUNDANCES_GAS},
-      {GAL_V_DISK_MASS_STELLAR, GAL_V_SPHEROID_MASS_STELLAR},
-      {GAL_V_DISK_ABUNDANCES_STELLAR, GAL_V_SPHEROID_ABUNDANCES_STELLAR},
-      {GAL_V_DISK_MASS_STELLAR_FORMED, GAL_V_SPHEROID_MASS_STELLAR_FORMED},
-      {GAL_V_DISK_ANGULAR_MOMENTUM, GAL_V_SPHEROID_ANGULAR_MOMENTUM},
-      {GAL_V_SPHEROID_MASS_GAS, GAL_V_SPHEROID_MASS_GAS},
-      {GAL_V_SPHEROID_ABUNDANCES_GAS, GAL_V_SPHEROID_ABUNDANCES_GAS},
-      {GAL_V_SPHEROID_MASS_STELLAR, GAL_V_SPHEROID_MASS_STELLAR},
-      {GAL_V_SPHEROID_ABUNDANCES_STELLAR, GAL_V_SPHEROID_ABUNDANCES_STELLAR},
-      {GAL_V_SPHEROID_MASS_STELLAR_FORMED, GAL_V_SPHEROID_MASS_STELLAR_FORMED},
-      {GAL_V_SPHEROID_ANGULAR_MOMENTUM, GAL_V_SPHEROID_ANGULAR_MOMENTUM},
-      // the satellite's circumgalactic reservoirs and black hole
-      {GAL_V_HOT_MASS, GAL_V_HOT_MASS},
-      {GAL_V_HOT_ABUNDANCES, GAL_V_HOT_ABUNDANCES},
-      {GAL_V_HOT_ANGULAR_MOMENTUM, GAL_V_HOT_ANGULAR_MOMENTUM},
-      {GAL_V_HOT_OUTFLOWED_MASS, GAL_V_HOT_OUTFLOWED_MASS},
-      {GAL_V_HOT_OUTFLOWED_ANGULAR_MOMENTUM, GAL_V_HOT_OUTFLOWED_ANGULAR_MOMENTUM},
-      {GAL_V_HOT_OUTFLOWED_ABUNDANCES, GAL_V_HOT_OUTFLOWED_ABUNDANCES},
-      {GAL_V_HOT_UNACCRETED_MASS, GAL_V_HOT_UNACCRETED_MASS},
-      {GAL_V_HOT_UNACCRETED_ABUNDANCES, GAL_V_HOT_UNACCRETED_ABUNDANCES},
-      {GAL_V_HOT_ENERGY_RADIATED, GAL_V_HOT_ENERGY_RADIATED},
-      {GAL_V_BH_MASS, GAL_V_BH_MASS},
-  };
-  for (const Move& m : sat_moves) {
-    Y(m.to, b) += Y(m.from, a);
-    Y(m.from, a) = 0.0;
-  }
-  // the satellite's stripped gas joins the host's outflowed reservoir, as
-  // hot_halo/standard/_class.F90:532-597 moves it after each satellite
-  // evolve (here at the merger: mass and metals; its angular momentum,
-  // M_stripped J_host/M_host there, is not carried)
-  Y(GAL_V_HOT_OUTFLOWED_MASS, b) += Y(GAL_V_HOT_STRIPPED_MASS, a);
-  Y(GAL_V_HOT_STRIPPED_MASS, a) = 0.0;
-  Y(GAL_V_HOT_OUTFLOWED_ABUNDANCES, b) += Y(GAL_V_HOT_STRIPPED_ABUNDANCES, a);
-  Y(GAL_V_HOT_STRIPPED_ABUNDANCES, a) = 0.0;
-  if (major) {
-    const Move host_moves[] = {
-        {GAL_V_DISK_MASS_GAS, GAL_V_SPHEROID_MASS_GAS},
-        {GAL_V_DISK_ABUNDANCES_GAS, GAL_V_SPHEROID_ABUNDANCES_GAS},
-        {GAL_V_DISK_MASS_STELLAR, GAL_V_SPHEROID_MASS_STELLAR},
-        {GAL_V_DISK_ABUNDANCES_STELLAR, GAL_V_SPHEROID_ABUNDANCES_STELLAR},
-        {GAL_V_DISK_MASS_STELLAR_FORMED, GAL_V_SPHEROID_MASS_STELLAR_FORMED},
-        {GAL_V_DISK_ANGULAR_MOMENTUM, GAL_V_SPHEROID_ANGULAR_MOMENTUM},
-    };
-    for (const Move& m : host_moves) {
-      Y(m.to, b) += Y(m.from, b);
-      Y(m.from, b) = 0.0;
-    }
-  }
+  apply_event(kind[i], (size_t)from[i], (size_t)to[i], ld, y, h_step, major_ratio);
 }
 
 // Compact copies of a wave's non-satellite evolve operations for the
@@ -354,6 +290,123 @@ static int tree_evolve(gal_ode_batch* h, const gal_schedule* sched, int ld, doub
   return rc;
 }
 
+// The schedule as an operation graph (events.hpp) run by one persistent
+// launch of the integrator: no wave waits for the slowest interval of the
+// one before it, only for the operations on its own slots.
+static int tree_evolve_graph(gal_ode_batch* h, const gal_schedule* sched, int ld, double* y, double* scale,
+                             int* status, double major_ratio, void* stream) {
+  if (!h || !sched || ld < 1 || ld > h->n_max || !y || !status) return GAL_E_INVALID;
+  if ((h->flags & GAL_FLAG_SCALES_PROVIDED) && !scale) return GAL_E_INVALID;
+  int64_t n_ops = 0;
+  int n_waves = 0;
+  if (gal_schedule_size(sched, &n_ops, &n_waves) != GAL_OK) return GAL_E_INVALID;
+  if (n_ops == 0) return GAL_OK;
+  std::vector<gal_schedule_op> ops((size_t)n_ops);
+  gal_schedule_ops(sched, ops.data());
+  // Operations that change state; becoming a satellite changes none and
+  // orders only its own node, so the operations either side of it on that
+  // node follow each other directly.
+  int64_t n64 = 0;
+  for (const gal_schedule_op& op : ops) {
+    if (op.a < 0 || op.a >= ld || op.b >= ld) return GAL_E_INVALID;
+    if (op.kind == GAL_OP_PROMOTE || op.kind == GAL_OP_MERGE_GALAXY) {
+      if (op.b < 0 || op.b == op.a) return GAL_E_INVALID;
+      ++n64;
+    } else if (op.kind == GAL_OP_EVOLVE) {
+      ++n64;
+    } else if (op.kind != GAL_OP_BECOME_SATELLITE) {
+      return GAL_E_INVALID;
+    }
+  }
+  if (n64 > INT32_MAX) return GAL_E_INVALID;
+  const int n = (int)n64;
+  std::vector<int> kind(n), sa(n), sb(n), next_a(n, -1), next_b(n, -1), npred(n, 0), ready(n, -1);
+  std::vector<double> t0(n, 0.0), t1(n, 0.0), np((size_t)GAL_NPARAM_NODE * n, 0.0);
+  std::vector<int> last((size_t)ld, -1);   // latest operation on each slot
+  int k = 0;
+  for (const gal_schedule_op& op : ops) {
+    if (op.kind == GAL_OP_BECOME_SATELLITE) continue;
+    const bool event = op.kind != GAL_OP_EVOLVE;
+    kind[k] = op.kind;
+    sa[k] = op.a;
+    sb[k] = event ? op.b : -1;
+    if (!event) {
+      t0[k] = op.t0;
+      t1[k] = op.t1;
+      for (int j = 0; j < GAL_NPARAM_NODE; ++j) np[(size_t)j * n + k] = op.node_par[j];
+    }
+    const int touched[2] = {sa[k], sb[k]};
+    for (int slot : touched) {
+      if (slot < 0) continue;
+      const int p = last[(size_t)slot];
+      if (p >= 0) {
+        (sa[p] == slot ? next_a[p] : next_b[p]) = k;
+        ++npred[k];
+      }
+      last[(size_t)slot] = k;
+    }
+    ++k;
+  }
+  unsigned int r0 = 0;
+  for (int i = 0; i < n; ++i)
+    if (npred[i] == 0) ready[r0++] = i;
+  hipStream_t s = (hipStream_t)stream;
+  bool ok = true;
+  int* d_kind = upload(kind, s, ok);
+  int* d_a = upload(sa, s, ok);
+  int* d_b = upload(sb, s, ok);
+  int* d_next_a = upload(next_a, s, ok);
+  int* d_next_b = upload(next_b, s, ok);
+  int* d_npred = upload(npred, s, ok);
+  int* d_ready = upload(ready, s, ok);
+  double* d_t0 = upload(t0, s, ok);
+  double* d_t1 = upload(t1, s, ok);
+  double* d_np = upload(np, s, ok);
+  const std::vector<unsigned int> tail0{r0};
+  const std::vector<int> err0{0};
+  unsigned int* d_tail = upload(tail0, s, ok);
+  int* d_err = upload(err0, s, ok);
+  double* d_h = nullptr;
+  ok = ok && hipMalloc(&d_h, sizeof(double) * ld) == hipSuccess;
+  int rc = ok ? GAL_OK : GAL_E_NOMEM;
+  if (rc == GAL_OK) {
+    hipLaunchKernelGGL(fill_kernel, dim3((ld + 255) / 256), dim3(256), 0, s, ld, -1.0, d_h);
+    if (hipGetLastError() != hipSuccess) rc = GAL_E_HIP;
+  }
+  if (rc == GAL_OK) {
+    gal::OpGraph g;
+    g.n = n;
+    g.kind = d_kind;
+    g.a = d_a;
+    g.b = d_b;
+    g.next_a = d_next_a;
+    g.next_b = d_next_b;
+    g.t0 = d_t0;
+    g.t1 = d_t1;
+    g.np = d_np;
+    g.npred = d_npred;
+    g.ready = d_ready;
+    g.tail = d_tail;
+    g.major_ratio = major_ratio;
+    rc = gal_ode_evolve_graph(h, ld, y, d_h, scale, status, g, d_err, s);
+  }
+  if (hipStreamSynchronize(s) != hipSuccess && rc == GAL_OK) rc = GAL_E_HIP;
+  if (rc == GAL_OK) {
+    int err = 0;
+    unsigned int appended = 0;
+    if (hipMemcpy(&err, d_err, sizeof(err), hipMemcpyDeviceToHost) != hipSuccess ||
+        hipMemcpy(&appended, d_tail, sizeof(appended), hipMemcpyDeviceToHost) != hipSuccess) {
+      rc = GAL_E_HIP;
+    } else if (err != 0 || appended != (unsigned int)n) {
+      fprintf(stderr, "gal_tree_evolve_d: the operation graph stopped with %u of %d operations ready\n", appended, n);
+      rc = GAL_E_HIP;
+    }
+  }
+  hipFree(d_kind); hipFree(d_a); hipFree(d_b); hipFree(d_next_a); hipFree(d_next_b); hipFree(d_npred);
+  hipFree(d_ready); hipFree(d_t0); hipFree(d_t1); hipFree(d_np); hipFree(d_tail); hipFree(d_err); hipFree(d_h);
+  return rc;
+}
+
 extern "C" int gal_node_events_h(int n_events, const int* kind, const int* node, const int* target, int n_nodes,
                                  double* y, double* h_step, double major_ratio) {
   if (n_events < 0 || n_nodes < 1 || (n_events > 0 && (!kind || !node || !target || !y || !h_step)))
@@ -394,6 +447,11 @@ extern "C" int gal_node_events_h(int n_events, const int* kind, const int* node,
 
 extern "C" int gal_tree_evolve_d(gal_ode_batch* h, const gal_schedule* sched, int ld, double* y, double* scale,
                                  int* status, double major_ratio, void* stream) {
+  return tree_evolve_graph(h, sched, ld, y, scale, status, major_ratio, stream);
+}
+
+extern "C" int gal_tree_evolve_waves_d(gal_ode_batch* h, const gal_schedule* sched, int ld, double* y, double* scale,
+                                       int* status, double major_ratio, void* stream) {
   return tree_evolve(h, sched, ld, y, scale, status, major_ratio, stream, nullptr);
 }
 

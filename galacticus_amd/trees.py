@@ -129,6 +129,7 @@ def _bind(L):
     L.gal_forest_initial_state.argtypes = [vp, ctypes.POINTER(Cosmology), ctypes.POINTER(WalkParams),
                                            ctypes.c_double, ctypes.c_double, c_int, dp]
     L.gal_tree_evolve_d.argtypes = [vp, vp, c_int, vp, vp, vp, ctypes.c_double, vp]
+    L.gal_tree_evolve_waves_d.argtypes = [vp, vp, c_int, vp, vp, vp, ctypes.c_double, vp]
     L.gal_node_events_h.argtypes = [c_int, ip, ip, ip, c_int, dp, dp, ctypes.c_double]
     L.gal_tree_evolve_surrogate_d.argtypes = [vp, vp, vp, ctypes.POINTER(_lib.Gate), vp, c_int, vp, vp, vp,
                                               ctypes.c_double, ctypes.POINTER(ctypes.c_int64), vp]
@@ -137,7 +138,7 @@ def _bind(L):
     for name in ("gal_forest_build_pch", "gal_forest_from_arrays", "gal_forest_size", "gal_forest_arrays",
                  "gal_forest_destroy", "gal_cosmology_sigma", "gal_cosmology_delta_critical", "gal_schedule_build",
                  "gal_schedule_size", "gal_schedule_ops", "gal_schedule_destroy", "gal_forest_initial_state",
-                 "gal_tree_evolve_d", "gal_tree_evolve_surrogate_d", "gal_tree_evolve_record_d",
+                 "gal_tree_evolve_d", "gal_tree_evolve_waves_d", "gal_tree_evolve_surrogate_d", "gal_tree_evolve_record_d",
                  "gal_node_events_h"):
         getattr(L, name).restype = c_int
     L._trees_bound = True
@@ -337,7 +338,7 @@ class Schedule:
             pass
 
 
-def evolve_forests(evolvers, schedules, ys, streams, major_ratio=0.25):
+def evolve_forests(evolvers, schedules, ys, streams, major_ratio=0.25, executor="graph"):
     """Evolve independent forests concurrently: forest k on evolver k and
     stream streams[k], each driven by its own host thread (the library call
     releases the interpreter lock), so one forest's waves overlap the
@@ -351,7 +352,8 @@ def evolve_forests(evolvers, schedules, ys, streams, major_ratio=0.25):
 
     def run(k):
         try:
-            out[k] = evolve_forest(evolvers[k], schedules[k], ys[k], major_ratio=major_ratio, stream=streams[k])
+            out[k] = evolve_forest(evolvers[k], schedules[k], ys[k], major_ratio=major_ratio, stream=streams[k],
+                                   executor=executor)
         except Exception as e:  # reported after every thread has joined
             err.append(e)
     threads = [threading.Thread(target=run, args=(k,)) for k in range(len(schedules))]
@@ -365,11 +367,15 @@ def evolve_forests(evolvers, schedules, ys, streams, major_ratio=0.25):
 
 
 def evolve_forest(evolver, schedule: Schedule, y, scale=None, status=None, major_ratio=0.25, stream=None,
-                  surrogate=None):
+                  surrogate=None, executor="graph"):
     """Run the schedule on device state y (torch float64, (30, n_nodes));
-    returns the per-node status tensor.  surrogate: a SurrogateEvolver whose
-    network and gate take the evolve operations of non-satellite halos
-    (gal_tree_evolve_surrogate_d); its count of accepted intervals is left in
+    returns the per-node status tensor.  executor: "graph" (one persistent
+    launch; each operation waits only for the operations before it on its
+    own nodes: gal_tree_evolve_d) or "waves" (wave after wave:
+    gal_tree_evolve_waves_d); both give the same state bit for bit.
+    surrogate: a SurrogateEvolver whose network and gate take the evolve
+    operations of non-satellite halos (gal_tree_evolve_surrogate_d, wave
+    after wave); its count of accepted intervals is left in
     surrogate.tree_accepted."""
     import torch
     n = int(y.shape[1])
@@ -380,9 +386,12 @@ def evolve_forest(evolver, schedule: Schedule, y, scale=None, status=None, major
     L = _bind(evolver.L)
     s = ctypes.c_void_p(0 if stream is None else stream.cuda_stream)
     if surrogate is None:
-        _lib.check(L.gal_tree_evolve_d(evolver.handle, schedule.handle, n, ctypes.c_void_p(y.data_ptr()),
-                                       ctypes.c_void_p(scale.data_ptr()), ctypes.c_void_p(status.data_ptr()),
-                                       float(major_ratio), s), "gal_tree_evolve_d")
+        if executor not in ("graph", "waves"):
+            raise ValueError(f"executor must be 'graph' or 'waves', not {executor!r}")
+        name = "gal_tree_evolve_d" if executor == "graph" else "gal_tree_evolve_waves_d"
+        _lib.check(getattr(L, name)(evolver.handle, schedule.handle, n, ctypes.c_void_p(y.data_ptr()),
+                                    ctypes.c_void_p(scale.data_ptr()), ctypes.c_void_p(status.data_ptr()),
+                                    float(major_ratio), s), name)
         return status
     n_acc = ctypes.c_int64(0)
     g = surrogate.gate.c_struct()

@@ -22,7 +22,7 @@
  *    one more than the latest wave of any earlier operation on the same
  *    node, so running wave after wave reproduces the sequential order
  *    operation for operation.
- * gal_tree_evolve_d (gal_tree_exec.hip) runs a schedule on the GPU.
+ * gal_tree_evolve_d (tree_exec.hip) runs a schedule on the GPU.
  */
 #ifndef GAL_TREE_H
 #define GAL_TREE_H
@@ -110,14 +110,28 @@ int gal_forest_initial_state(const gal_forest* f, const gal_cosmology* cosmo, co
                              double metal_fraction, double black_hole_seed, int ld, double* y);
 
 /* ---- execution on the GPU ------------------------------------------------ */
-/* Run a schedule wave by wave: each wave's EVOLVE operations as one batched
-   evolve (gal_ode.h semantics) of the listed nodes, then its event
-   operations.  y, scale: SoA [GAL_NVAR*ld] over the forest's nodes (device);
-   status: [ld] (device).  Merger mass movements follow mergerMassMovementsSimple
-   (satellites/merging/mass_movements/simple.F90:208-290) with
-   major_ratio the major-merger mass ratio. */
+/* Run a schedule (the work of mergerTreeEvolverStandard%evolve,
+   merger_trees/evolver/standard.F90:291-1035, for every tree of the forest)
+   as an operation graph in one persistent launch of the integrator: each
+   operation waits only for the previous operations on the nodes it touches;
+   idle lanes claim ready operations, evolve intervals with gal_ode.h
+   semantics, apply events inline and release what follows.  y, scale: SoA
+   [GAL_NVAR*ld] over the forest's nodes (device); status: [ld] (device), the
+   status of each node's latest evolve.  Merger mass movements follow
+   mergerMassMovementsSimple (satellites/merging/mass_movements/simple.F90:
+   208-290) with major_ratio the major-merger mass ratio.  The handle's
+   wall-time limit, if set, covers the whole forest.  Returns GAL_E_HIP if the
+   graph stops making progress (no operation becomes ready for a minute). */
 int gal_tree_evolve_d(gal_ode_batch* h, const gal_schedule* s, int ld, double* y, double* scale, int* status,
                       double major_ratio, void* stream);
+
+/* The same schedule wave by wave: each wave's EVOLVE operations as one
+   batched evolve of the listed nodes, then its event operations; a wave
+   waits for the slowest interval of the wave before it.  The result equals
+   gal_tree_evolve_d's bit for bit (the handle's wall-time limit, if set,
+   applies to each wave's launch). */
+int gal_tree_evolve_waves_d(gal_ode_batch* h, const gal_schedule* s, int ld, double* y, double* scale,
+                            int* status, double major_ratio, void* stream);
 
 /* Apply node events to host arrays (copied to the device and back): each
    event is GAL_OP_PROMOTE (node's galaxy moves into its parent target:

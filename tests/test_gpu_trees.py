@@ -4,7 +4,9 @@
   closedBoxTree.xml) run through the forest reader, the walk and the wave
   executor reproduces the closed-box known answers
   (testSuite/test-reproducibility.py:47-57) at the root.
-* A Parkinson-Cole-Helly forest run wave by wave on the GPU equals the same
+* The operation-graph executor equals the wave executor bit for bit at
+  every launch shape.
+* A Parkinson-Cole-Helly forest run on the GPU equals the same
   schedule applied one operation at a time on the CPU with the oracle
   integrator and the oracle's merger mass movements (oracle/trees.py): median
   relative difference below 1e-9 (both sides take the same steps almost
@@ -172,6 +174,54 @@ def test_concurrent_forests_equal_one_forest(torch_cuda, cooling_table):
     torch.cuda.synchronize()
     assert np.array_equal(np.concatenate([y.cpu().numpy() for y in ys], axis=1), yd.cpu().numpy())
     assert np.array_equal(np.concatenate([s.cpu().numpy() for s in sts]), st)
+
+
+@pytest.mark.parametrize("launch", [(64, 1, 1), (64, 2, 1), (64, 4, 1), (64, 1, 2), (64, 2, 2)])
+def test_graph_executor_equals_wave_executor(torch_cuda, cooling_table, launch):
+    """The operation-graph executor (one persistent launch, each operation
+    waiting only for the earlier operations on its own nodes) ends in the
+    state, step sizes, scales and statuses of the wave executor bit for bit,
+    at every launch shape: every node sees the same operations in the same
+    order with the same inputs."""
+    torch = torch_cuda
+    from galacticus_amd.evolver import NodeEvolverBatched
+    physics = gp.physics_params()
+    cosmo = tr.cosmology(physics)
+    forest = tr.Forest.build_pch(24, np.geomspace(1e11, 1e13, 24), T0, cosmo, tr.pch_params(2e10, T_Z5), seed=7)
+    wp = tr.walk_params(T0, merging_multiplier=0.2)
+    sched = tr.Schedule(forest, cosmo, wp)
+    kinds = np.bincount(sched.ops["kind"], minlength=4)
+    print("launch", launch, "nodes", forest.n_nodes, "ops", sched.n_ops, "waves", sched.n_waves, "kinds", kinds,
+          flush=True)
+    assert kinds[tr.OP_PROMOTE] > 0 and kinds[tr.OP_MERGE_GALAXY] > 0 and kinds[tr.OP_BECOME_SATELLITE] > 0
+    y0 = forest.initial_state(cosmo, wp)
+    ev = NodeEvolverBatched(forest.n_nodes, physics, cooling_table, odeToleranceAbsolute=1e-6,
+                            odeToleranceRelative=1e-3)
+    ev.set_launch(launch[0], launch[1], rhs_waves=launch[2])
+    out = {}
+    for executor in ("waves", "graph"):
+        yd = torch.from_numpy(y0.copy()).cuda()
+        sd = torch.zeros_like(yd)
+        st = tr.evolve_forest(ev, sched, yd, scale=sd, executor=executor)
+        torch.cuda.synchronize()
+        out[executor] = (yd.cpu().numpy(), sd.cpu().numpy(), st.cpu().numpy())
+    evolved = np.unique(sched.ops["a"][sched.ops["kind"] == tr.OP_EVOLVE])
+    assert (out["graph"][2][evolved] == 0).all()
+    for a, b in zip(out["waves"], out["graph"]):
+        assert np.array_equal(a, b)
+
+
+def test_graph_executor_rejects_unknown_executor(torch_cuda, cooling_table):
+    torch = torch_cuda
+    from galacticus_amd.evolver import NodeEvolverBatched
+    physics = gp.physics_params()
+    cosmo = tr.cosmology(physics)
+    forest = tr.Forest.build_pch(2, np.array([1e11, 2e11]), T0, cosmo, tr.pch_params(2e10, T_Z5), seed=1)
+    wp = tr.walk_params(T0)
+    ev = NodeEvolverBatched(forest.n_nodes, physics, cooling_table)
+    yd = torch.from_numpy(forest.initial_state(cosmo, wp)).cuda()
+    with pytest.raises(ValueError):
+        tr.evolve_forest(ev, tr.Schedule(forest, cosmo, wp), yd, executor="levels")
 
 
 def test_surrogate_route_rejecting_everything_equals_plain_forest(torch_cuda, cooling_table):
