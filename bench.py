@@ -471,16 +471,29 @@ def _forest_run(physics, table, tol_abs, tol_rel, n_trees, log_m, z_start, label
     k = np.bincount(sched.ops["kind"], minlength=4)
     out = {"workload": label, "nodes": forest.n_nodes, "operations": sched.n_ops, "waves": sched.n_waves,
            "evolve_intervals": int(k[0]), "promotions": int(k[1]), "halo_mergers": int(k[2]),
-           "galaxy_mergers": int(k[3]), "node_gyr": gyr, "gpu_seconds": gpu_s, "node_gyr_per_s": gyr / gpu_s,
+           "galaxy_mergers": int(k[3]), "node_gyr": gyr, "executor": "operation graph, one persistent launch",
+           "launch": list(ev.launch()), "gpu_seconds": gpu_s, "node_gyr_per_s": gyr / gpu_s,
            "host_build_seconds": build_s, "host_schedule_seconds": sched_s, "statuses_ok": bool((st == 0).all())}
     log(f"trees ({label}): {forest.n_nodes} nodes, {sched.n_ops} operations in {sched.n_waves} waves, "
-        f"GPU {gpu_s:.3f} s ({gyr / gpu_s:.4g} node-Gyr/s)")
-    del holder, ev
+        f"GPU {gpu_s:.3f} s ({gyr / gpu_s:.4g} node-Gyr/s) as one operation graph")
+    # the same schedule wave by wave: every wave waits for its slowest interval
+    hold_w = {}
+
+    def run_waves():
+        y = torch.from_numpy(y0).cuda()
+        hold_w["st"] = tr.evolve_forest(ev, sched, y, executor="waves")
+        hold_w["y"] = y
+    waves_s, _ = _timed(run_waves, reps=1)
+    same_w = bool(np.array_equal(hold_w["y"].cpu().numpy(), yg))
+    out["wave_executor"] = {"gpu_seconds": waves_s, "node_gyr_per_s": gyr / waves_s,
+                            "identical_to_graph": same_w}
+    log(f"trees ({label}): wave by wave {waves_s:.3f} s ({gyr / waves_s:.4g} node-Gyr/s), identical states: {same_w}")
+    del holder, hold_w, ev
     # the same trees as F independent forests, evolved concurrently over S
-    # streams (forest k on stream k mod S): a wave of one no longer waits for
-    # the slowest interval of all the others.  Smaller forests have shorter
-    # waves, but when every wave fills the GPU, more launches in flight only
-    # interfere: each (F, S) in `split` is timed and the fastest is kept.
+    # streams (forest k on stream k mod S), each as its own operation graph:
+    # each (F, S) in `split` is timed and the fastest is kept.  One graph
+    # already lets every operation start as soon as its own nodes allow, so
+    # the default runs none.
     for n_forests, n_streams in split:
         parts = forest.split(n_forests)
         scheds = [tr.Schedule(f, cosmo, wp) for f, _ in parts]
@@ -603,7 +616,7 @@ def main():
     ap.add_argument("--cpu-seconds", type=float, default=15.0)
     ap.add_argument("--no-surrogate", action="store_true", help="skip the surrogate training/inference legs")
     ap.add_argument("--no-trees", action="store_true", help="skip the merger-tree leg")
-    ap.add_argument("--tree-splits", default="4,8",
+    ap.add_argument("--tree-splits", default="",
                     help="comma-separated splits of a tree workload into concurrent forests, F (F forests on F "
                          "streams) or FxS (F forests on S streams); the fastest is reported")
     ap.add_argument("--large-trees", type=int, default=256,

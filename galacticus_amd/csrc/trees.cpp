@@ -417,7 +417,7 @@ struct Walk {
   gal_walk_params w;
   std::vector<gal_schedule_op>& ops;
   int64_t lo, hi;  // this tree's nodes
-  std::vector<int32_t> parent, first_child, sibling, first_sat;
+  std::vector<int32_t> parent, first_child, sibling, first_sat, last_sat;
   std::vector<char> is_sat, alive;
   std::vector<double> time, merge_time;
   std::vector<double> par;  // node parameters (5 per node) of the node's current halo
@@ -427,7 +427,7 @@ struct Walk {
        int64_t l, int64_t h)
       : f(ff), c(cc), w(ww), ops(o), lo(l), hi(h) {
     const int64_t n = hi - lo;
-    parent.resize(n); first_child.resize(n); sibling.resize(n); first_sat.assign(n, -1);
+    parent.resize(n); first_child.resize(n); sibling.resize(n); first_sat.assign(n, -1); last_sat.assign(n, -1);
     is_sat.assign(n, 0); alive.assign(n, 1); time.resize(n); merge_time.assign(n, kInf); par.assign((size_t)GAL_NPARAM_NODE * n, 0.0);
     for (int64_t i = 0; i < n; ++i) {
       auto loc = [&](int32_t g) { return g < 0 ? -1 : (int32_t)(g - lo); };
@@ -484,27 +484,40 @@ struct Walk {
     }
     return parent[n];
   }
+  // Satellite lists are singly linked through sibling[] in arrival order,
+  // with the tail kept so that appending (and handing a whole list to a new
+  // host, as a promotion does for thousands of satellites) costs no walk.
   void append_satellite(int32_t host, int32_t s) {
     sibling[s] = -1;
     parent[s] = host;
-    if (first_sat[host] < 0) { first_sat[host] = s; return; }
-    int32_t k = first_sat[host];
-    while (sibling[k] >= 0) k = sibling[k];
-    sibling[k] = s;
+    if (first_sat[host] < 0) first_sat[host] = s;
+    else sibling[last_sat[host]] = s;
+    last_sat[host] = s;
   }
   void remove_satellite(int32_t host, int32_t s) {
-    if (first_sat[host] == s) { first_sat[host] = sibling[s]; sibling[s] = -1; return; }
+    if (first_sat[host] == s) {
+      first_sat[host] = sibling[s];
+      if (first_sat[host] < 0) last_sat[host] = -1;
+      sibling[s] = -1;
+      return;
+    }
     int32_t k = first_sat[host];
     while (k >= 0 && sibling[k] != s) k = sibling[k];
-    if (k >= 0) sibling[k] = sibling[s];
+    if (k >= 0) {
+      sibling[k] = sibling[s];
+      if (last_sat[host] == s) last_sat[host] = k;
+    }
     sibling[s] = -1;
   }
+  // Every satellite of `from` joins the end of `to`'s list, in order.
   void move_satellites(int32_t from, int32_t to) {
-    while (first_sat[from] >= 0) {
-      const int32_t s = first_sat[from];
-      first_sat[from] = sibling[s];
-      append_satellite(to, s);
-    }
+    const int32_t head = first_sat[from];
+    if (head < 0) return;
+    for (int32_t s = head; s >= 0; s = sibling[s]) parent[s] = to;
+    if (first_sat[to] < 0) first_sat[to] = head;
+    else sibling[last_sat[to]] = head;
+    last_sat[to] = last_sat[from];
+    first_sat[from] = last_sat[from] = -1;
   }
   void emit(int kind, int32_t a, int32_t b, double t0, double t1, const double* np) {
     gal_schedule_op op{};
@@ -845,6 +858,9 @@ int gal_schedule_build(const gal_forest* f, const gal_cosmology* cosmo, const ga
     });
     for (int t = 0; t < f->n_trees; ++t)
       if (!ok[t]) return GAL_E_INVALID;
+    size_t total = 0;
+    for (const auto& p : parts) total += p.size();
+    s->ops.reserve(total);
     for (auto& p : parts) {
       s->ops.insert(s->ops.end(), p.begin(), p.end());
       std::vector<gal_schedule_op>().swap(p);
