@@ -91,23 +91,21 @@ __device__ inline void apply_event(int kind, size_t a, size_t b, int ld, double*
 }
 
 // A forest's operations as a dependency graph for one persistent launch of
-// the integrator.  Operation k touches slot a[k] (and b[k] for events); it
-// waits on the previous operation on each slot it touches, and next_a[k],
-// next_b[k] are the operations that follow it on those slots (-1: none).
-// npred[k] counts the links still unfinished; an operation whose count
-// reaches zero is appended to ready[] (tail counts the appends), from which
-// idle lanes claim entries in order.  Every operation is appended exactly
-// once, so ready[] has n entries; entries not yet written hold -1.
+// the integrator.  ops is the device copy of the schedule's operations,
+// indexed as in the schedule; the graph holds n of them (becoming a
+// satellite changes no state and is left out).  Operation k touches slot
+// ops[k].a (and ops[k].b for events); it waits on the previous operation on
+// each slot it touches, and next_a[k], next_b[k] are the operations that
+// follow it on those slots (-1: none).  npred[k] counts the links still
+// unfinished; an operation whose count reaches zero is appended to ready[]
+// (tail counts the appends), from which idle lanes claim entries in order.
+// Every operation of the graph is appended exactly once, so ready[] has n
+// entries; entries not yet written hold -1.
 struct OpGraph {
-  int n = 0;                    // operations
-  const int* kind = nullptr;    // GAL_OP_EVOLVE, GAL_OP_PROMOTE or GAL_OP_MERGE_GALAXY
-  const int* a = nullptr;
-  const int* b = nullptr;
+  int n = 0;                               // operations in the graph
+  const gal_schedule_op* ops = nullptr;    // the schedule's operations (device)
   const int* next_a = nullptr;
   const int* next_b = nullptr;
-  const double* t0 = nullptr;   // evolve intervals
-  const double* t1 = nullptr;
-  const double* np = nullptr;   // halo parameters, SoA with leading dimension n
   int* npred = nullptr;
   int* ready = nullptr;
   unsigned int* tail = nullptr;

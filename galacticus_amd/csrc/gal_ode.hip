@@ -44,6 +44,7 @@
 #include <mutex>
 
 #include "../../include/gal_ode.h"
+#include "../../include/gal_tree.h"
 #include "constants.hpp"
 #include "physics.hpp"
 #include "integrator.hpp"
@@ -358,16 +359,17 @@ __global__ __launch_bounds__(BLOCK * WAVES, 1) void evolve_kernel(EvolveArgs a) 
             const int k = graph_ready_entry(a.g, pend);
             if (k >= 0) {
               pend = -1;
-              const int kind = a.g.kind[k];
+              const gal_schedule_op* o = a.g.ops + k;
+              const int kind = o->kind;
               if (kind != GAL_OP_EVOLVE) {
-                apply_event(kind, (size_t)a.g.a[k], (size_t)a.g.b[k], a.ld, a.y, a.h_step, a.g.major_ratio);
+                apply_event(kind, (size_t)o->a, (size_t)o->b, a.ld, a.y, a.h_step, a.g.major_ratio);
                 graph_release(a.g, k);
               } else {
                 op = k;
-                node = a.g.a[k];
-                np = load_node_par(a.g.np, a.g.n, k);
-                t0 = a.g.t0[k];
-                t1 = a.g.t1[k];
+                node = o->a;
+                np = load_node_par(o->node_par, 1, 0);
+                t0 = o->t0;
+                t1 = o->t1;
                 begun = true;
               }
             } else {
@@ -1427,13 +1429,12 @@ int gal_ode_evolve_graph(gal_ode_batch* h, int ld, double* y, double* h_step, do
                          const gal::OpGraph& g, int* graph_error, hipStream_t s) {
   if (!h || ld < 1 || ld > h->n_max || g.n < 0 || !graph_error) return GAL_E_INVALID;
   if (g.n == 0) return GAL_OK;
-  if (!g.kind || !g.a || !g.b || !g.next_a || !g.next_b || !g.t0 || !g.t1 || !g.np || !g.npred || !g.ready ||
-      !g.tail)
-    return GAL_E_INVALID;
+  if (!g.ops || !g.next_a || !g.next_b || !g.npred || !g.ready || !g.tail) return GAL_E_INVALID;
   EvolveArgs a;
   // the per-node arrays of the node mode are unused: intervals and halo
   // parameters come with each operation
-  int rc = make_args(h, ld, ld, y, g.np, g.t0, g.t1, h_step, scale, status, nullptr, a);
+  const double* unused = &g.ops->t0;
+  int rc = make_args(h, ld, ld, y, unused, unused, unused, h_step, scale, status, nullptr, a);
   if (rc != GAL_OK) return rc;
   a.g = g;
   a.graph_error = graph_error;

@@ -18,6 +18,8 @@
 #include <algorithm>
 #include <cstdio>
 #include <cstdint>
+#include <memory>
+#include <mutex>
 #include <vector>
 
 #include "../../include/gal_ode.h"
@@ -26,6 +28,7 @@
 #include "physics.hpp"
 #include "batch.hpp"
 #include "events.hpp"
+#include "schedule.hpp"
 
 using namespace gal;
 
@@ -290,120 +293,183 @@ static int tree_evolve(gal_ode_batch* h, const gal_schedule* sched, int ld, doub
   return rc;
 }
 
-// The schedule as an operation graph (events.hpp) run by one persistent
-// launch of the integrator: no wave waits for the slowest interval of the
-// one before it, only for the operations on its own slots.
-static int tree_evolve_graph(gal_ode_batch* h, const gal_schedule* sched, int ld, double* y, double* scale,
-                             int* status, double major_ratio, void* stream) {
-  if (!h || !sched || ld < 1 || ld > h->n_max || !y || !status) return GAL_E_INVALID;
-  if ((h->flags & GAL_FLAG_SCALES_PROVIDED) && !scale) return GAL_E_INVALID;
-  int64_t n_ops = 0;
-  int n_waves = 0;
-  if (gal_schedule_size(sched, &n_ops, &n_waves) != GAL_OK) return GAL_E_INVALID;
-  if (n_ops == 0) return GAL_OK;
-  std::vector<gal_schedule_op> ops((size_t)n_ops);
-  gal_schedule_ops(sched, ops.data());
-  // Operations that change state; becoming a satellite changes none and
-  // orders only its own node, so the operations either side of it on that
-  // node follow each other directly.
-  int64_t n64 = 0;
+namespace {
+
+// ---- the operation graph --------------------------------------------------
+// A schedule's operation graph on one device, built on the first call on
+// that device and kept with the schedule: the operations themselves, the
+// successor links and the pristine predecessor counts and ready list, plus
+// the state of a run (counts, ready list, counters, per-slot step sizes),
+// which every call resets on the device.  Calls that share a schedule on one
+// device therefore must not overlap.
+struct DeviceGraph {
+  int device = -1;
+  int max_slot = -1;  // highest slot an operation names
+  int n_all = 0;      // operations of the schedule
+  int n = 0;          // operations in the graph
+  unsigned int r0 = 0;  // operations ready at the start
+  gal_schedule_op* ops = nullptr;
+  int *next_a = nullptr, *next_b = nullptr, *npred0 = nullptr, *ready0 = nullptr;
+  int *npred = nullptr, *ready = nullptr, *err = nullptr;
+  unsigned int* tail = nullptr;
+  double* h = nullptr;
+  int h_cap = 0;
+  ~DeviceGraph() {
+    int cur = -1;
+    const bool switch_device = hipGetDevice(&cur) == hipSuccess && cur != device && device >= 0;
+    if (switch_device) (void)hipSetDevice(device);
+    (void)hipFree(ops); (void)hipFree(next_a); (void)hipFree(next_b); (void)hipFree(npred0);
+    (void)hipFree(ready0); (void)hipFree(npred); (void)hipFree(ready); (void)hipFree(err); (void)hipFree(tail);
+    (void)hipFree(h);
+    if (switch_device) (void)hipSetDevice(cur);
+  }
+};
+
+void delete_device_graph(void* p) { delete static_cast<DeviceGraph*>(p); }
+
+// Links and initial ready list over the schedule's operations (host, once).
+int build_device_graph(const gal_schedule& sc, int device, hipStream_t s, DeviceGraph& g) {
+  const std::vector<gal_schedule_op>& ops = sc.ops;
+  if (ops.size() > (size_t)INT32_MAX) return GAL_E_INVALID;
+  const int n_all = (int)ops.size();
+  int max_slot = -1;
   for (const gal_schedule_op& op : ops) {
-    if (op.a < 0 || op.a >= ld || op.b >= ld) return GAL_E_INVALID;
+    if (op.a < 0) return GAL_E_INVALID;
     if (op.kind == GAL_OP_PROMOTE || op.kind == GAL_OP_MERGE_GALAXY) {
       if (op.b < 0 || op.b == op.a) return GAL_E_INVALID;
-      ++n64;
-    } else if (op.kind == GAL_OP_EVOLVE) {
-      ++n64;
-    } else if (op.kind != GAL_OP_BECOME_SATELLITE) {
+    } else if (op.kind != GAL_OP_EVOLVE && op.kind != GAL_OP_BECOME_SATELLITE) {
       return GAL_E_INVALID;
     }
+    max_slot = std::max(max_slot, std::max(op.a, op.b));
   }
-  if (n64 > INT32_MAX) return GAL_E_INVALID;
-  const int n = (int)n64;
-  std::vector<int> kind(n), sa(n), sb(n), next_a(n, -1), next_b(n, -1), npred(n, 0), ready(n, -1);
-  std::vector<double> t0(n, 0.0), t1(n, 0.0), np((size_t)GAL_NPARAM_NODE * n, 0.0);
-  std::vector<int> last((size_t)ld, -1);   // latest operation on each slot
-  int k = 0;
-  for (const gal_schedule_op& op : ops) {
+  std::vector<int> next_a(n_all, -1), next_b(n_all, -1), npred(n_all, 0), ready;
+  std::vector<int> last((size_t)max_slot + 1, -1);   // latest operation on each slot
+  int n = 0;
+  for (int k = 0; k < n_all; ++k) {
+    const gal_schedule_op& op = ops[(size_t)k];
+    // becoming a satellite changes no state and orders only its own node:
+    // the operations either side of it on that node follow each other
     if (op.kind == GAL_OP_BECOME_SATELLITE) continue;
-    const bool event = op.kind != GAL_OP_EVOLVE;
-    kind[k] = op.kind;
-    sa[k] = op.a;
-    sb[k] = event ? op.b : -1;
-    if (!event) {
-      t0[k] = op.t0;
-      t1[k] = op.t1;
-      for (int j = 0; j < GAL_NPARAM_NODE; ++j) np[(size_t)j * n + k] = op.node_par[j];
-    }
-    const int touched[2] = {sa[k], sb[k]};
+    ++n;
+    const int touched[2] = {op.a, op.kind == GAL_OP_EVOLVE ? -1 : op.b};
     for (int slot : touched) {
       if (slot < 0) continue;
       const int p = last[(size_t)slot];
       if (p >= 0) {
-        (sa[p] == slot ? next_a[p] : next_b[p]) = k;
-        ++npred[k];
+        (ops[(size_t)p].a == slot ? next_a[(size_t)p] : next_b[(size_t)p]) = k;
+        ++npred[(size_t)k];
       }
       last[(size_t)slot] = k;
     }
-    ++k;
   }
+  ready.assign((size_t)std::max(n, 1), -1);
   unsigned int r0 = 0;
-  for (int i = 0; i < n; ++i)
-    if (npred[i] == 0) ready[r0++] = i;
-  hipStream_t s = (hipStream_t)stream;
+  for (int k = 0; k < n_all; ++k)
+    if (ops[(size_t)k].kind != GAL_OP_BECOME_SATELLITE && npred[(size_t)k] == 0) ready[r0++] = k;
+  g.device = device;
+  g.max_slot = max_slot;
+  g.n_all = n_all;
+  g.n = n;
+  g.r0 = r0;
   bool ok = true;
-  int* d_kind = upload(kind, s, ok);
-  int* d_a = upload(sa, s, ok);
-  int* d_b = upload(sb, s, ok);
-  int* d_next_a = upload(next_a, s, ok);
-  int* d_next_b = upload(next_b, s, ok);
-  int* d_npred = upload(npred, s, ok);
-  int* d_ready = upload(ready, s, ok);
-  double* d_t0 = upload(t0, s, ok);
-  double* d_t1 = upload(t1, s, ok);
-  double* d_np = upload(np, s, ok);
-  const std::vector<unsigned int> tail0{r0};
-  const std::vector<int> err0{0};
-  unsigned int* d_tail = upload(tail0, s, ok);
-  int* d_err = upload(err0, s, ok);
-  double* d_h = nullptr;
-  ok = ok && hipMalloc(&d_h, sizeof(double) * ld) == hipSuccess;
-  int rc = ok ? GAL_OK : GAL_E_NOMEM;
-  if (rc == GAL_OK) {
-    hipLaunchKernelGGL(fill_kernel, dim3((ld + 255) / 256), dim3(256), 0, s, ld, -1.0, d_h);
-    if (hipGetLastError() != hipSuccess) rc = GAL_E_HIP;
+  g.ops = upload(ops, s, ok);
+  g.next_a = upload(next_a, s, ok);
+  g.next_b = upload(next_b, s, ok);
+  g.npred0 = upload(npred, s, ok);
+  g.ready0 = upload(ready, s, ok);
+  ok = ok && hipMalloc(&g.npred, sizeof(int) * (size_t)n_all) == hipSuccess &&
+       hipMalloc(&g.ready, sizeof(int) * ready.size()) == hipSuccess &&
+       hipMalloc(&g.err, sizeof(int)) == hipSuccess && hipMalloc(&g.tail, sizeof(unsigned int)) == hipSuccess;
+  // the host vectors are released on return: the copies must have landed
+  if (hipStreamSynchronize(s) != hipSuccess) return GAL_E_HIP;
+  return ok ? GAL_OK : GAL_E_NOMEM;
+}
+
+// The schedule's graph on the current device, built on first use.
+int device_graph(gal_schedule* sc, hipStream_t s, DeviceGraph** out) {
+  int device = 0;
+  if (hipGetDevice(&device) != hipSuccess) return GAL_E_HIP;
+  std::lock_guard<std::mutex> lock(sc->attach_mutex);
+  for (auto& at : sc->attachments)
+    if (at.first == device) {
+      *out = static_cast<DeviceGraph*>(at.second.get());
+      return GAL_OK;
+    }
+  auto g = std::make_unique<DeviceGraph>();
+  const int rc = build_device_graph(*sc, device, s, *g);
+  if (rc != GAL_OK) return rc;
+  *out = g.get();
+  sc->attachments.emplace_back(device, gal_schedule::Attachment(g.release(), delete_device_graph));
+  return GAL_OK;
+}
+
+// Reset a run: predecessor counts, ready list, counters and step sizes.
+__global__ void graph_reset_kernel(int n_all, const int* __restrict__ npred0, int* __restrict__ npred, int n,
+                                   const int* __restrict__ ready0, int* __restrict__ ready, unsigned int r0,
+                                   unsigned int* __restrict__ tail, int* __restrict__ err, int ld,
+                                   double* __restrict__ h) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n_all) npred[i] = npred0[i];
+  if (i < n) ready[i] = ready0[i];
+  if (i < ld) h[i] = -1.0;
+  if (i == 0) {
+    *tail = r0;
+    *err = 0;
   }
-  if (rc == GAL_OK) {
-    gal::OpGraph g;
-    g.n = n;
-    g.kind = d_kind;
-    g.a = d_a;
-    g.b = d_b;
-    g.next_a = d_next_a;
-    g.next_b = d_next_b;
-    g.t0 = d_t0;
-    g.t1 = d_t1;
-    g.np = d_np;
-    g.npred = d_npred;
-    g.ready = d_ready;
-    g.tail = d_tail;
-    g.major_ratio = major_ratio;
-    rc = gal_ode_evolve_graph(h, ld, y, d_h, scale, status, g, d_err, s);
+}
+
+}  // namespace
+
+// The schedule as an operation graph (events.hpp) run by one persistent
+// launch of the integrator: no wave waits for the slowest interval of the
+// one before it, only for the operations on its own slots.
+static int tree_evolve_graph(gal_ode_batch* h, const gal_schedule* sched_c, int ld, double* y, double* scale,
+                             int* status, double major_ratio, void* stream) {
+  if (!h || !sched_c || ld < 1 || ld > h->n_max || !y || !status) return GAL_E_INVALID;
+  if ((h->flags & GAL_FLAG_SCALES_PROVIDED) && !scale) return GAL_E_INVALID;
+  gal_schedule* sched = const_cast<gal_schedule*>(sched_c);   // the graph is a cache kept with it
+  if (sched->ops.empty()) return GAL_OK;
+  hipStream_t s = (hipStream_t)stream;
+  DeviceGraph* dg = nullptr;
+  int rc = device_graph(sched, s, &dg);
+  if (rc != GAL_OK) return rc;
+  if (dg->max_slot >= ld) return GAL_E_INVALID;
+  if (dg->n == 0) return GAL_OK;
+  if (dg->h_cap < ld) {
+    (void)hipFree(dg->h);
+    dg->h = nullptr;
+    dg->h_cap = 0;
+    if (hipMalloc(&dg->h, sizeof(double) * (size_t)ld) != hipSuccess) return GAL_E_NOMEM;
+    dg->h_cap = ld;
   }
+  const int span = std::max(std::max(dg->n_all, dg->n), ld);
+  hipLaunchKernelGGL(graph_reset_kernel, dim3((unsigned)((span + 255) / 256)), dim3(256), 0, s, dg->n_all,
+                     (const int*)dg->npred0, dg->npred, dg->n, (const int*)dg->ready0, dg->ready, dg->r0, dg->tail,
+                     dg->err, ld, dg->h);
+  if (hipGetLastError() != hipSuccess) return GAL_E_HIP;
+  gal::OpGraph g;
+  g.n = dg->n;
+  g.ops = dg->ops;
+  g.next_a = dg->next_a;
+  g.next_b = dg->next_b;
+  g.npred = dg->npred;
+  g.ready = dg->ready;
+  g.tail = dg->tail;
+  g.major_ratio = major_ratio;
+  rc = gal_ode_evolve_graph(h, ld, y, dg->h, scale, status, g, dg->err, s);
   if (hipStreamSynchronize(s) != hipSuccess && rc == GAL_OK) rc = GAL_E_HIP;
   if (rc == GAL_OK) {
     int err = 0;
     unsigned int appended = 0;
-    if (hipMemcpy(&err, d_err, sizeof(err), hipMemcpyDeviceToHost) != hipSuccess ||
-        hipMemcpy(&appended, d_tail, sizeof(appended), hipMemcpyDeviceToHost) != hipSuccess) {
+    if (hipMemcpy(&err, dg->err, sizeof(err), hipMemcpyDeviceToHost) != hipSuccess ||
+        hipMemcpy(&appended, dg->tail, sizeof(appended), hipMemcpyDeviceToHost) != hipSuccess) {
       rc = GAL_E_HIP;
-    } else if (err != 0 || appended != (unsigned int)n) {
-      fprintf(stderr, "gal_tree_evolve_d: the operation graph stopped with %u of %d operations ready\n", appended, n);
+    } else if (err != 0 || appended != (unsigned int)dg->n) {
+      fprintf(stderr, "gal_tree_evolve_d: the operation graph stopped with %u of %d operations ready\n", appended,
+              dg->n);
       rc = GAL_E_HIP;
     }
   }
-  hipFree(d_kind); hipFree(d_a); hipFree(d_b); hipFree(d_next_a); hipFree(d_next_b); hipFree(d_npred);
-  hipFree(d_ready); hipFree(d_t0); hipFree(d_t1); hipFree(d_np); hipFree(d_tail); hipFree(d_err); hipFree(d_h);
   return rc;
 }
 

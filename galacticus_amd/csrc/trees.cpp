@@ -17,6 +17,7 @@
 
 #include "../../include/gal_tree.h"
 #include "constants.hpp"
+#include "schedule.hpp"
 
 // Host arithmetic is evaluated as written (no fused multiply-add), so that
 // times and halo parameters are reproducible by any IEEE double evaluation of
@@ -729,10 +730,6 @@ void parallel_trees(int n, F fn) {
 
 }  // namespace
 
-struct gal_schedule {
-  std::vector<gal_schedule_op> ops;
-  int n_waves = 0;
-};
 
 extern "C" {
 

@@ -46,6 +46,9 @@ import numpy as np
 
 NVAR = 30
 NPARAM_NODE = 7
+# Rows of the per-node parameters (enum gal_node_param in include/gal_ode.h).
+NP_TIME_REFERENCE, NP_MASS_REFERENCE, NP_MASS_RATE, NP_SPIN_RATE, NP_IS_SATELLITE, NP_RAM_PRESSURE_FORCE, \
+    NP_HOST_VELOCITY_VIRIAL = range(NPARAM_NODE)
 
 # Indices mirror enum gal_param in include/gal_ode.h.
 PARAM_NAMES = [

@@ -79,6 +79,8 @@ def test_parameter_layout_matches_host_tables():
     assert len(vals) - 1 == len(gp.PARAM_NAMES)
     node = _enum_values("gal_node_param")
     assert node[-1] == ("GAL_NPARAM_NODE", gp.NPARAM_NODE)
+    for name, value in node[:-1]:
+        assert getattr(gp, name[len("GAL_"):]) == value
 
 
 def test_flags_and_statuses_match():

@@ -340,3 +340,104 @@ def test_64_tree_workload_against_the_cpu_tree_runner(torch_cuda, cooling_table)
     mc = yc[V["DISK_MASS_STELLAR"], r] + yc[V["SPHEROID_MASS_STELLAR"], r]
     print("root stellar mass max rel", float(np.max(np.abs(mg - mc) / mc)), flush=True)
     assert np.allclose(mg, mc, rtol=1e-4, atol=0)
+
+
+BARYONS = ["HOT_MASS", "HOT_OUTFLOWED_MASS", "HOT_UNACCRETED_MASS", "HOT_STRIPPED_MASS", "DISK_MASS_GAS",
+           "DISK_MASS_STELLAR", "SPHEROID_MASS_GAS", "SPHEROID_MASS_STELLAR", "BH_MASS"]
+METALS = ["HOT_ABUNDANCES", "HOT_OUTFLOWED_ABUNDANCES", "HOT_UNACCRETED_ABUNDANCES", "HOT_STRIPPED_ABUNDANCES",
+          "DISK_ABUNDANCES_GAS", "DISK_ABUNDANCES_STELLAR", "SPHEROID_ABUNDANCES_GAS", "SPHEROID_ABUNDANCES_STELLAR"]
+
+
+def test_large_forest_sample_budgets_and_cpu_agreement(torch_cuda, cooling_table):
+    """The first 8 trees of the 4096-tree large-forest workload (PCH, seed 0,
+    roots log-uniform 1e11-1e14 Msun, resolution 1e9 Msun, z=8 -> 0; the
+    builder seeds each tree by its index, so these are that workload's first
+    trees) as one operation graph on the GPU:
+      * every evolve succeeds;
+      * per tree, baryons and metals close over the nodes still alive at z=0
+        (never promoted away or merged): the sources are the leaves' initial
+        galaxies, accretion f_b dM/dt dt onto halos that are not satellites
+        and the yield of new stars; the sinks are the black holes' radiated
+        and jet energy, at most eff/(1 - eff) <= 0.75 of their growth, for
+        baryons and metals alike (the expulsion by excess heating and the
+        clipping of negative reservoirs are off so that every step conserves
+        exactly); the residual is held to 1e-7 of the tree's budget;
+      * the trees with the smallest schedules equal the CPU oracle's tree
+        runner on the same operations to the ODE tolerance, with the z=0
+        stellar masses of their roots within 1e-4."""
+    torch = torch_cuda
+    from galacticus_amd.evolver import NodeEvolverBatched
+    physics = gp.physics_params(**{"CGMCoolingHeating.excessHeatDrivesOutflow": 0.0,
+                                   "postStepZeroNegativeMasses": 0.0})
+    cosmo = tr.cosmology(physics)
+    t_z8 = float(wl.scale_factor_to_cosmic_time(1.0 / 9.0))
+    n_trees = 8
+    roots = 10.0 ** np.random.Generator(np.random.PCG64(0)).uniform(11.0, 14.0, 4096)[:n_trees]
+    forest = tr.Forest.build_pch(n_trees, roots, T0, cosmo, tr.pch_params(1e9, t_z8), seed=0)
+    wp = tr.walk_params(T0)
+    sched = tr.Schedule(forest, cosmo, wp)
+    ops = sched.ops
+    tree = forest.tree_of_node()
+    print("roots", roots, "nodes", forest.n_nodes, "operations", sched.n_ops, "waves", sched.n_waves, flush=True)
+    y0 = forest.initial_state(cosmo, wp)
+    ev = NodeEvolverBatched(forest.n_nodes, physics, cooling_table, odeToleranceAbsolute=1e-6,
+                            odeToleranceRelative=1e-3)
+    yd = torch.from_numpy(y0.copy()).cuda()
+    sd = torch.zeros_like(yd)
+    st = tr.evolve_forest(ev, sched, yd, scale=sd).cpu().numpy()
+    yg, sc = yd.cpu().numpy(), sd.cpu().numpy()
+    evolved = np.unique(ops["a"][ops["kind"] == tr.OP_EVOLVE])
+    assert (st[evolved] == 0).all()
+    # budgets per tree over the nodes alive at the end
+    gone = np.zeros(forest.n_nodes, bool)
+    gone[ops["a"][(ops["kind"] == tr.OP_PROMOTE) | (ops["kind"] == tr.OP_MERGE_GALAXY)]] = True
+    alive = ~gone
+    ev_ops = ops[ops["kind"] == tr.OP_EVOLVE]
+    fb = physics[gp.P["OmegaBaryon"]] / physics[gp.P["OmegaMatter"]]
+    yld = physics[gp.P["metalYield"]]
+    acc_op = np.where(ev_ops["node_par"][:, gp.NP_IS_SATELLITE] != 0, 0.0,
+                      fb * ev_ops["node_par"][:, gp.NP_MASS_RATE] * (ev_ops["t1"] - ev_ops["t0"]))
+    acc = np.bincount(tree[ev_ops["a"]], weights=acc_op, minlength=n_trees)
+
+    def per_tree(y, names, mask):
+        return np.bincount(tree[mask], weights=sum(y[V[k]] for k in names)[mask], minlength=n_trees)
+    every = np.ones(forest.n_nodes, bool)
+    b0, b1 = per_tree(y0, BARYONS, every), per_tree(yg, BARYONS, alive)
+    bh0, bh1 = per_tree(y0, ["BH_MASS"], every), per_tree(yg, ["BH_MASS"], alive)
+    dbh = np.maximum(bh1 - bh0, 0.0)
+    loss = acc - (b1 - b0)
+    scale = b0 + acc
+    print("baryon loss / budget", loss / scale, "bh growth / budget", dbh / scale, flush=True)
+    assert (loss >= -1e-7 * scale).all()
+    assert (loss <= 0.75 * dbh + 1e-7 * scale).all()
+    formed = per_tree(yg, ["DISK_MASS_STELLAR_FORMED", "SPHEROID_MASS_STELLAR_FORMED"], alive) - \
+        per_tree(y0, ["DISK_MASS_STELLAR_FORMED", "SPHEROID_MASS_STELLAR_FORMED"], every)
+    z0, z1 = per_tree(y0, METALS, every), per_tree(yg, METALS, alive)
+    zres = (z1 - z0) - yld * formed
+    zscale = z0 + yld * np.abs(formed)
+    print("metal residual / budget", zres / zscale, flush=True)
+    assert (zres <= 1e-7 * zscale).all()
+    assert (-zres <= 2.5 * dbh + 1e-7 * zscale).all()
+    # the smallest trees against the CPU tree runner
+    n_ops_tree = np.bincount(tree[ops["a"]], minlength=n_trees)
+    small = np.nonzero(n_ops_tree <= 200000)[0]
+    print("operations per tree", n_ops_tree, "compared", small, flush=True)
+    assert len(small) >= 3
+    sel = np.isin(tree[ops["a"]], small)
+    yc, _, stc, bad = oracle.tree_run(ops[sel], y0, -np.ones(forest.n_nodes), tree, n_trees, physics, cooling_table,
+                                      1e-6, 1e-3, gp.FLAG_REUSE_STEP_SIZE, 0.25, root_tol_rel=1e-15, nthreads=8)
+    assert bad == 0
+    nodes = np.isin(tree, small)
+    mass = [V[k] for k in ("HOT_MASS", "DISK_MASS_GAS", "DISK_MASS_STELLAR", "SPHEROID_MASS_GAS",
+                           "SPHEROID_MASS_STELLAR", "BH_MASS")]
+    g, c, s = yg[mass][:, nodes], yc[mass][:, nodes], sc[mass][:, nodes]
+    ref = np.abs(c)
+    big = ref > 1e-6 * np.abs(yc[V["HOT_MASS"]][nodes]).max()
+    rel = np.abs(g - c)[big] / ref[big]
+    worst = (np.abs(g - c) / np.maximum(1e-3 * ref + 1e-6 * np.abs(s), 1e-300)).max()
+    print("median rel", np.median(rel), "max |dy|/tol", worst, flush=True)
+    assert np.median(rel) < 1e-9 and worst < 50.0
+    r = forest.roots()[small]
+    mg = yg[V["DISK_MASS_STELLAR"], r] + yg[V["SPHEROID_MASS_STELLAR"], r]
+    mc = yc[V["DISK_MASS_STELLAR"], r] + yc[V["SPHEROID_MASS_STELLAR"], r]
+    assert np.allclose(mg, mc, rtol=1e-4, atol=0)
