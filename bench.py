@@ -120,6 +120,22 @@ def cpu_baseline(batch, physics, table, tol_abs, tol_rel, flags, target_seconds=
 MASS_HEADS = ("DISK_MASS_STELLAR", "SPHEROID_MASS_STELLAR", "DISK_MASS_GAS", "SPHEROID_MASS_GAS", "HOT_MASS", "BH_MASS")
 
 
+_STREAMS = []
+
+
+def stream_pool(n):
+    """n streams from one pool for the whole run.  Every stream holds a
+    hardware queue for the rest of the process (destroying it does not give
+    the queue back), and with more than about 24 queues in a process the
+    hardware scheduler time-slices them: a persistent tree launch then takes
+    1.5-1.8x as long.  The legs therefore share the integrator leg's streams
+    instead of creating their own."""
+    import torch
+    while len(_STREAMS) < n:
+        _STREAMS.append(torch.cuda.Stream())
+    return _STREAMS[:n]
+
+
 def _timed(fn, reps=2, all_times=None):
     import torch
     best = None
@@ -306,7 +322,7 @@ def pipelined_surrogate(ev, mlp, norm, gate, test, h0, launch, physics, table, t
     params = mlp.get_params()
     mlps = [mlp] + [sg.SurrogateMLP(8192, params=params) for _ in range(n_streams - 1)]
     ses = [sg.SurrogateEvolver(e, m, norm, gate) for e, m in zip(evs, mlps)]
-    streams = [torch.cuda.Stream() for _ in range(n_streams)]
+    streams = stream_pool(n_streams)
     status = [torch.zeros(n, dtype=torch.int32, device=test.t0.device) for _ in range(n_streams)]
     accepted = [torch.ones(n, dtype=torch.int32, device=test.t0.device) for _ in range(n_streams)]
     out = {}
@@ -513,7 +529,7 @@ def _forest_run(physics, table, tol_abs, tol_rel, n_trees, log_m, z_start, label
         scheds = [tr.Schedule(f, cosmo, wp) for f, _ in parts]
         evs = [NodeEvolverBatched(f.n_nodes, physics, table, odeToleranceAbsolute=tol_abs,
                                   odeToleranceRelative=tol_rel) for f, _ in parts]
-        pool = [torch.cuda.Stream() for _ in range(min(n_streams, len(parts)))]
+        pool = stream_pool(min(n_streams, len(parts)))
         streams = [pool[k % len(pool)] for k in range(len(parts))]
         hold = {}
 
@@ -706,7 +722,7 @@ def main():
     # One stream per evolver handle, created once: the runtime maps streams
     # to its hardware queues in creation order, so tuning and the timed region
     # share the same queues.
-    streams = [torch.cuda.Stream() for _ in range(n_streams)]
+    streams = stream_pool(n_streams)
     tuning = {}
     shapes = ((64, 1, 1), (64, 2, 1), (64, 4, 1), (64, 1, 2), (64, 2, 2))
     if args.launch != "auto":
