@@ -33,8 +33,12 @@ sys.path.insert(0, ROOT)
 # concurrent forests): with 8 queues the four forest streams shared queues
 # (64 trees in 0.29 s), with 16 they did not (0.17 s).  Read when the HIP
 # runtime starts, so it is set before torch is.
-# Up to 32 evolve streams: 32 queues (the most this pool allows) keep every
-# one of 16 on a queue of its own and let 32 share them in pairs.
+# Up to 32 evolve streams, each on a queue of its own: 32 queues (the most
+# this pool allows).  With 22, the 20 timed batches ran at 14.5 ms per step
+# instead of 11.1; with 20, two of them shared a queue (19.6 ms).  A process
+# keeps a queue for every stream it ever used, and beyond about 24 the
+# hardware scheduler time-slices them, which makes persistent tree launches
+# 1.5-1.8x slower: the tree leg runs before the others create their streams.
 if int(os.environ.get("GPU_MAX_HW_QUEUES", "4")) < 32:
     os.environ["GPU_MAX_HW_QUEUES"] = "32"
 
@@ -457,15 +461,12 @@ def chained_histories(ev, se, physics, args, n=20000):
             "rk_seconds": times["rk"], "surrogate_seconds": times["surrogate"]}
 
 
-def _forest_run(physics, table, tol_abs, tol_rel, n_trees, log_m, z_start, label, cpu_threads, cpu=True, split=(),
-                surrogate=None, waves=True):
-    """Build a PCH forest, record its walk, evolve it on the GPU as one
-    operation graph (and, with waves, wave by wave for comparison) and (cpu)
-    with the CPU oracle's tree runner over cpu_threads threads; compare the
-    z=0 stellar masses of the roots."""
-    import torch
+def _forest(physics, n_trees, log_m, z_start):
+    """A PCH forest of the tree workloads (seed 0, resolution 1e9 Msun, roots
+    log-uniform in 10^log_m Msun at z=0, from z_start), its walk and initial
+    state; returns (forest, schedule, y0, build seconds, schedule seconds,
+    cosmology, walk parameters)."""
     from galacticus_amd import trees as tr, workloads as wl
-    from galacticus_amd.evolver import NodeEvolverBatched
     cosmo = tr.cosmology(physics)
     t0 = float(wl.scale_factor_to_cosmic_time(1.0))
     t_start = float(wl.scale_factor_to_cosmic_time(1.0 / (1.0 + z_start)))
@@ -478,7 +479,19 @@ def _forest_run(physics, table, tol_abs, tol_rel, n_trees, log_m, z_start, label
     t = time.perf_counter()
     sched = tr.Schedule(forest, cosmo, wp)
     sched_s = time.perf_counter() - t
-    y0 = forest.initial_state(cosmo, wp)
+    return forest, sched, forest.initial_state(cosmo, wp), build_s, sched_s, cosmo, wp
+
+
+def _forest_run(physics, table, tol_abs, tol_rel, n_trees, log_m, z_start, label, cpu_threads, cpu=True, split=(),
+                waves=True):
+    """Build a PCH forest, record its walk, evolve it on the GPU as one
+    operation graph (and, with waves, wave by wave for comparison) and (cpu)
+    with the CPU oracle's tree runner over cpu_threads threads; compare the
+    z=0 stellar masses of the roots."""
+    import torch
+    from galacticus_amd import trees as tr
+    from galacticus_amd.evolver import NodeEvolverBatched
+    forest, sched, y0, build_s, sched_s, cosmo, wp = _forest(physics, n_trees, log_m, z_start)
     ev = NodeEvolverBatched(forest.n_nodes, physics, table, odeToleranceAbsolute=tol_abs, odeToleranceRelative=tol_rel)
     holder = {}
 
@@ -549,31 +562,6 @@ def _forest_run(physics, table, tol_abs, tol_rel, n_trees, log_m, z_start, label
             out["gpu_seconds"], out["node_gyr_per_s"] = split_s, gyr / split_s
             out["split"] = rec
         del hold, evs
-    y_sur = None
-    if surrogate is not None:
-        # the same forest with the trained surrogate in front of the
-        # integrator for the intervals of non-satellite halos
-        from galacticus_amd.surrogate import SurrogateEvolver
-        ev_t = NodeEvolverBatched(forest.n_nodes, physics, table, odeToleranceAbsolute=tol_abs,
-                                  odeToleranceRelative=tol_rel)
-        se_t = SurrogateEvolver(ev_t, surrogate.mlp, surrogate.norm, surrogate.gate)
-        hold = {}
-
-        def run_sur():
-            y = torch.from_numpy(y0).cuda()
-            hold["st"] = tr.evolve_forest(ev_t, sched, y, surrogate=se_t)
-            hold["y"] = y
-        sur_s, _ = _timed(run_sur)
-        y_sur = hold["y"].cpu().numpy()
-        n_ev = int(k[0])
-        n_ns = int(((sched.ops["kind"] == 0) & (sched.ops["node_par"][:, 4] == 0)).sum())
-        out["surrogate"] = {"gpu_seconds": sur_s, "node_gyr_per_s": gyr / sur_s,
-                            "accepted_intervals": se_t.tree_accepted, "evolve_intervals": n_ev,
-                            "non_satellite_intervals": n_ns, "accepted_fraction": se_t.tree_accepted / max(n_ev, 1),
-                            "statuses_ok": bool((hold["st"].cpu().numpy() == 0).all())}
-        log(f"trees ({label}): with the surrogate {sur_s:.3f} s, {se_t.tree_accepted} of {n_ev} intervals "
-            f"accepted ({se_t.tree_accepted / max(n_ev, 1):.2f})")
-        del hold, ev_t
     if cpu:
         import oracle
         from galacticus_amd import params as gp
@@ -600,16 +588,55 @@ def _forest_run(physics, table, tol_abs, tol_rel, n_trees, log_m, z_start, label
                                          "smf_bins_0.25dex_count_moved": smf_diff, "roots": int(len(r))}
         log(f"trees ({label}): CPU {cpu_s:.2f} s on {cpu_threads} threads, GPU/CPU {cpu_s / out['gpu_seconds']:.1f}x, "
             f"root M* median rel diff {np.median(rel):.2e}, SMF galaxies moved between bins {smf_diff}")
-        if y_sur is not None:
-            ms = y_sur[V["DISK_MASS_STELLAR"], r] + y_sur[V["SPHEROID_MASS_STELLAR"], r]
-            rs = np.abs(ms - mc) / np.maximum(mc, 1e-300)
-            hs = np.histogram(np.log10(np.maximum(ms, 1e-30)), bins)[0]
-            out["surrogate"]["z0_stellar_mass_vs_cpu"] = {
-                "median_relative": float(np.median(rs)), "p90_relative": float(np.quantile(rs, 0.9)),
-                "max_relative": float(rs.max()), "smf_bins_0.25dex_count_moved": int(np.abs(hs - hc).sum()),
-                "roots": int(len(r))}
-            log(f"trees ({label}): surrogate root M* vs CPU median rel {np.median(rs):.2e}, p90 "
-                f"{np.quantile(rs, 0.9):.2e}, SMF galaxies moved {int(np.abs(hs - hc).sum())}")
+    return out
+
+
+def surrogate_forest_route(se, physics, table, tol_abs, tol_rel, n_trees):
+    """The larger forest of the tree leg with the trained surrogate in front
+    of the integrator for the intervals of non-satellite halos (per-wave
+    route), against the same forest's operation graph without it: time, and
+    z=0 stellar masses of the roots."""
+    import torch
+    from galacticus_amd import trees as tr
+    from galacticus_amd.evolver import NodeEvolverBatched
+    from galacticus_amd.surrogate import SurrogateEvolver
+    forest, sched, y0, _, _, _, _ = _forest(physics, n_trees, (11.0, 13.0), 8.0)
+    ev = NodeEvolverBatched(forest.n_nodes, physics, table, odeToleranceAbsolute=tol_abs, odeToleranceRelative=tol_rel)
+    se_t = SurrogateEvolver(ev, se.mlp, se.norm, se.gate)
+    hold = {}
+
+    def run(sur):
+        y = torch.from_numpy(y0).cuda()
+        hold["st"] = tr.evolve_forest(ev, sched, y, surrogate=sur)
+        hold["y"] = y
+    plain_s, _ = _timed(lambda: run(None))
+    y_plain = hold["y"].cpu().numpy()
+    sur_s, _ = _timed(lambda: run(se_t))
+    y_sur = hold["y"].cpu().numpy()
+    from galacticus_amd.workloads import V
+    r = forest.roots()
+    mp = y_plain[V["DISK_MASS_STELLAR"], r] + y_plain[V["SPHEROID_MASS_STELLAR"], r]
+    ms = y_sur[V["DISK_MASS_STELLAR"], r] + y_sur[V["SPHEROID_MASS_STELLAR"], r]
+    rel = np.abs(ms - mp) / np.maximum(mp, 1e-300)
+    bins = np.arange(6.0, 12.51, 0.25)
+    moved = int(np.abs(np.histogram(np.log10(np.maximum(ms, 1e-30)), bins)[0] -
+                       np.histogram(np.log10(np.maximum(mp, 1e-30)), bins)[0]).sum())
+    gyr = sched.node_gyr()
+    n_ev = int((sched.ops["kind"] == 0).sum())
+    n_ns = int(((sched.ops["kind"] == 0) & (sched.ops["node_par"][:, 4] == 0)).sum())
+    out = {"gpu_seconds": sur_s, "node_gyr_per_s": gyr / sur_s, "plain_graph_gpu_seconds_here": plain_s,
+           "accepted_intervals": se_t.tree_accepted, "evolve_intervals": n_ev, "non_satellite_intervals": n_ns,
+           "accepted_fraction": se_t.tree_accepted / max(n_ev, 1),
+           "statuses_ok": bool((hold["st"].cpu().numpy() == 0).all()),
+           "z0_stellar_mass_vs_plain_gpu": {"median_relative": float(np.median(rel)),
+                                            "p90_relative": float(np.quantile(rel, 0.9)),
+                                            "max_relative": float(rel.max()), "smf_bins_0.25dex_count_moved": moved,
+                                            "roots": int(len(r))},
+           "note": "run after the integrator and surrogate legs (their streams hold hardware queues); the plain "
+                   "operation graph is timed again beside it"}
+    log(f"trees ({n_trees} trees): with the surrogate {sur_s:.3f} s against the plain graph's {plain_s:.3f} s here, "
+        f"{se_t.tree_accepted} of {n_ev} intervals accepted; z=0 root M* vs plain median {np.median(rel):.2e}, "
+        f"p90 {np.quantile(rel, 0.9):.2e}, SMF galaxies moved {moved}")
     return out
 
 
@@ -627,7 +654,7 @@ def trees_leg(physics, table, tol_abs, tol_rel, args):
         out["large"] = _forest_run(physics, table, tol_abs, tol_rel, args.large_trees, (11.0, 13.0), 8.0,
                                    f"{args.large_trees} PCH trees, seed 0, roots log-uniform 1e11-1e13 Msun, "
                                    f"resolution 1e9 Msun, z=8 -> 0", threads, cpu=not args.no_cpu,
-                                   split=args.tree_splits, surrogate=getattr(args, "trained_surrogate", None))
+                                   split=args.tree_splits)
     if args.c5_trees > 0:
         # a sample of the 4096-tree large-forest workload: its first trees
         # (the builder seeds every tree by its index), the same roots'
@@ -698,6 +725,10 @@ def main():
     tol_abs, tol_rel = 1.0e-6, 1.0e-3
     physics = gp.physics_params()
     table = synthetic_cie_table()
+    # The tree leg runs first, on the null stream, before the integrator leg
+    # gives the process its 20 streams: persistent tree launches slow down
+    # 1.5-1.8x once a process holds more than about 24 hardware queues.
+    trees_out = None if args.no_trees else trees_leg(physics, table, tol_abs, tol_rel, args)
     batch = wl.isolated_halos(args.nodes, seed=args.seed, params=physics)
     n = batch.n
     node_gyr = batch.node_gyr
@@ -888,8 +919,11 @@ def main():
     del ys, hs
     if not args.no_surrogate:
         result["surrogate"] = surrogate_leg(physics, table, tol_abs, tol_rel, args)
-    if not args.no_trees:
-        result["trees"] = trees_leg(physics, table, tol_abs, tol_rel, args)
+    if trees_out is not None:
+        if not args.no_surrogate and args.large_trees > 0:
+            trees_out["large"]["surrogate"] = surrogate_forest_route(args.trained_surrogate, physics, table, tol_abs,
+                                                                     tol_rel, args.large_trees)
+        result["trees"] = trees_out
     if not args.no_cpu:
         cb = cpu_baseline(batch, physics, table, tol_abs, tol_rel, gp.FLAG_REUSE_STEP_SIZE, args.cpu_seconds)
         result["cpu_baseline"] = cb

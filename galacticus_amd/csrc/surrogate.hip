@@ -1050,8 +1050,10 @@ int gal_mlp_create(int batch, gal_mlp** out) {
     ok = hipMalloc(&m->d_z[l], sizeof(uint16_t) * kHid * B) == hipSuccess &&
          hipMalloc(&m->d_h[l], sizeof(uint16_t) * kHid * B) == hipSuccess;
   for (int l = 0; l < 2 && ok; ++l) ok = hipMalloc(&m->d_dz[l], sizeof(uint16_t) * kHid * B) == hipSuccess;
-  ok = ok && hipStreamCreateWithFlags(&m->stream, hipStreamNonBlocking) == hipSuccess &&
-       hipEventCreateWithFlags(&m->ev_in, hipEventDisableTiming) == hipSuccess &&
+  // the training stream is created on the first training call: every stream
+  // holds a hardware queue for the rest of the process, and inference-only
+  // handles never need one
+  ok = ok && hipEventCreateWithFlags(&m->ev_in, hipEventDisableTiming) == hipSuccess &&
        hipEventCreateWithFlags(&m->ev_out, hipEventDisableTiming) == hipSuccess;
   if (ok) {
     ok = hipMemset(m->d_param, 0, sizeof(float) * kParams) == hipSuccess &&
@@ -1198,6 +1200,7 @@ int gal_mlp_set_dataset(gal_mlp* m, int64_t n, const uint16_t* x, const float* t
 int gal_mlp_train_d(gal_mlp* m, int n_steps, int use_graph, float* loss_trace, void* stream) {
   if (!m || n_steps < 0 || !m->ds_x) return GAL_E_INVALID;
   if (n_steps == 0) return GAL_OK;
+  if (!m->stream) HIP_TRY(hipStreamCreateWithFlags(&m->stream, hipStreamNonBlocking));
   if (n_steps > m->loss_hist_cap) {
     release_graph(m);
     hipFree(m->d_loss_hist);
