@@ -21,7 +21,7 @@ EXPORTS = [
     "gal_ode_batch_set_physics", "gal_ode_batch_set_cooling_table", "gal_ode_batch_set_launch",
     "gal_ode_batch_evolve_d", "gal_ode_batch_evolve_h", "gal_ode_rhs_d", "gal_ode_hadjust_d",
     "gal_ode_batch_autotune", "gal_ode_evolve_static_d", "gal_ode_batch_set_rhs_waves", "gal_ode_batch_get_launch",
-    "gal_ode_batch_set_wall_limit",
+    "gal_ode_batch_set_wall_limit", "gal_ode_batch_set_graph_stall_limit",
 ]
 
 _lib = None
@@ -66,6 +66,7 @@ def load():
     L.gal_ode_batch_autotune.argtypes = [vp, c_int, c_int, vp, vp, vp, vp, vp, vp, vp, vp, ip, ip]
     L.gal_ode_batch_set_rhs_waves.argtypes = [vp, c_int]
     L.gal_ode_batch_set_wall_limit.argtypes = [vp, c_double]
+    L.gal_ode_batch_set_graph_stall_limit.argtypes = [vp, c_double]
     L.gal_ode_batch_get_launch.argtypes = [vp, ip, ip, ip]
     L.gal_ode_evolve_static_d.argtypes = [dp, c_int, dp, c_int, c_double, c_double, c_int, c_double, c_double,
                                           c_double, c_double, c_int, c_int, c_int, vp, vp, vp, vp, vp, vp, vp, vp,

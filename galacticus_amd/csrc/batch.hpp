@@ -21,6 +21,7 @@ struct gal_ode_batch {
   int rhs_waves = 2;          // waves per group of 64 nodes (2: the RHS split over two SIMDs)
   double wall_limit_s = 0.0;  // per-launch wall-time limit (0: none)
   double wall_clock_hz = 0.0; // frequency of the device's realtime counter
+  double graph_stall_s = 60.0;  // operation-graph executor: give up after this long without progress
   double params[GAL_NPARAM] = {};
   gal::DerivedConstants D{};
   gal::CoolingDesc cd{};

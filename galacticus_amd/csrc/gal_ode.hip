@@ -1058,6 +1058,12 @@ int gal_ode_batch_set_wall_limit(gal_ode_batch* h, double seconds) {
   return GAL_OK;
 }
 
+int gal_ode_batch_set_graph_stall_limit(gal_ode_batch* h, double seconds) {
+  if (!h || !(seconds == seconds)) return GAL_E_INVALID;
+  h->graph_stall_s = seconds > 0.0 ? seconds : 60.0;
+  return GAL_OK;
+}
+
 int gal_ode_batch_set_rhs_waves(gal_ode_batch* h, int waves) {
   if (!h || (waves != 1 && waves != 2)) return GAL_E_INVALID;
   h->rhs_waves = waves;
@@ -1439,8 +1445,9 @@ int gal_ode_evolve_graph(gal_ode_batch* h, int ld, double* y, double* h_step, do
   a.g = g;
   a.graph_error = graph_error;
   // a claimed entry may wait for the whole remaining critical path, but
-  // not for a minute in which no operation anywhere becomes ready
+  // not for the handle's stall limit (a minute by default) in which no
+  // operation anywhere becomes ready
   const double hz = h->wall_clock_hz > 0.0 ? h->wall_clock_hz : 1.0e8;
-  a.stall_ticks = (unsigned long long)(60.0 * hz);
+  a.stall_ticks = (unsigned long long)std::min(std::max(h->graph_stall_s * hz, 1.0), 1.0e18);
   return launch_evolve<true>(h, a, s);
 }

@@ -114,6 +114,13 @@ class NodeEvolverBatched:
         1025 (errorStatusXCPU) with their state as it came in."""
         _lib.check(self.L.gal_ode_batch_set_wall_limit(self.handle, float(seconds)), "gal_ode_batch_set_wall_limit")
 
+    def set_graph_stall_limit(self, seconds: float):
+        """Operation-graph executor of forests: lanes give up, and the call
+        fails, when no operation has become ready for this long (<= 0: the
+        default, 60 s)."""
+        _lib.check(self.L.gal_ode_batch_set_graph_stall_limit(self.handle, float(seconds)),
+                   "gal_ode_batch_set_graph_stall_limit")
+
     def launch(self):
         """(nodes per workgroup, workgroups per CU, waves per group of nodes)."""
         bn, bb, bw = ctypes.c_int(), ctypes.c_int(), ctypes.c_int()

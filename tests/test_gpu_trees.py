@@ -211,6 +211,36 @@ def test_graph_executor_equals_wave_executor(torch_cuda, cooling_table, launch):
         assert np.array_equal(a, b)
 
 
+def test_graph_executor_stall_limit_fails_loudly_and_recovers(torch_cuda, cooling_table):
+    """The graph executor's safety net, forced from the host: with a stall
+    limit far below one operation, lanes waiting on the ready list give up
+    (every kernel still exits: a lane that gives up leaves the loop, the
+    others finish their operations and then give up in turn), and the call
+    raises instead of returning a half-evolved forest.  With the default
+    limit the same handle and schedule then evolve the forest exactly as the
+    wave executor does."""
+    torch = torch_cuda
+    from galacticus_amd import _lib
+    from galacticus_amd.evolver import NodeEvolverBatched
+    physics = gp.physics_params()
+    cosmo = tr.cosmology(physics)
+    forest = tr.Forest.build_pch(8, np.geomspace(1e11, 1e12, 8), T0, cosmo, tr.pch_params(2e10, T_Z5), seed=6)
+    wp = tr.walk_params(T0, merging_multiplier=0.2)
+    sched = tr.Schedule(forest, cosmo, wp)
+    y0 = forest.initial_state(cosmo, wp)
+    ev = NodeEvolverBatched(forest.n_nodes, physics, cooling_table)
+    ev.set_graph_stall_limit(1e-9)
+    with pytest.raises(_lib.GalError):
+        tr.evolve_forest(ev, sched, torch.from_numpy(y0.copy()).cuda())
+    torch.cuda.synchronize()
+    ev.set_graph_stall_limit(0.0)
+    ya = torch.from_numpy(y0.copy()).cuda()
+    sa = tr.evolve_forest(ev, sched, ya)
+    yb = torch.from_numpy(y0.copy()).cuda()
+    sb = tr.evolve_forest(ev, sched, yb, executor="waves")
+    assert torch.equal(ya, yb) and torch.equal(sa, sb)
+
+
 def test_graph_executor_rejects_unknown_executor(torch_cuda, cooling_table):
     torch = torch_cuda
     from galacticus_amd.evolver import NodeEvolverBatched
