@@ -640,6 +640,92 @@ def surrogate_forest_route(se, physics, table, tol_abs, tol_rel, n_trees):
     return out
 
 
+def large_forest_stream(physics, table, tol_abs, tol_rel, n_trees, per_forest):
+    """The large-forest workload streamed: n_trees PCH trees (roots drawn as
+    the 4096-tree workload's, log-uniform 1e11-1e14 Msun, resolution 1e9
+    Msun, z=8 -> 0) as consecutive forests of per_forest trees (forest k from
+    builder seed k).  While the GPU evolves forest k as one operation graph,
+    a host thread builds and walks forest k+1 (the library calls release the
+    interpreter lock).  Reports the totals, the GPU and host time per forest
+    and the z=0 stellar-mass function of the roots."""
+    import threading
+    import torch
+    from galacticus_amd import trees as tr, workloads as wl
+    from galacticus_amd.evolver import NodeEvolverBatched
+    cosmo = tr.cosmology(physics)
+    t0 = float(wl.scale_factor_to_cosmic_time(1.0))
+    t8 = float(wl.scale_factor_to_cosmic_time(1.0 / 9.0))
+    wp = tr.walk_params(t0)
+    roots = 10.0 ** np.random.Generator(np.random.PCG64(0)).uniform(11.0, 14.0, 4096)[:n_trees]
+    n_forests = (n_trees + per_forest - 1) // per_forest
+
+    def prepare(k):
+        t = time.perf_counter()
+        f = tr.Forest.build_pch(min(per_forest, n_trees - k * per_forest), roots[k * per_forest:(k + 1) * per_forest],
+                                t0, cosmo, tr.pch_params(1e9, t8), seed=k)
+        sc = tr.Schedule(f, cosmo, wp)
+        return f, sc, f.initial_state(cosmo, wp), time.perf_counter() - t
+
+    totals = {"nodes": 0, "operations": 0, "node_gyr": 0.0, "gpu_seconds": 0.0, "host_seconds": 0.0}
+    per = []
+    ms_roots = []
+    ok = True
+    t_wall = time.perf_counter()
+    nxt = {}
+    th = None
+    cur = prepare(0)
+    for k in range(n_forests):
+        if k + 1 < n_forests:
+            th = threading.Thread(target=lambda j=k + 1: nxt.__setitem__("v", prepare(j)))
+            th.start()
+        f, sc, y0, host_s = cur
+        ev = NodeEvolverBatched(f.n_nodes, physics, table, odeToleranceAbsolute=tol_abs, odeToleranceRelative=tol_rel)
+        y = torch.from_numpy(y0).cuda()
+        torch.cuda.synchronize()
+        t = time.perf_counter()
+        st = tr.evolve_forest(ev, sc, y)
+        torch.cuda.synchronize()
+        gpu_s = time.perf_counter() - t
+        yh = y.cpu().numpy()
+        ok = ok and bool((st.cpu().numpy() == 0).all())
+        r = f.roots()
+        from galacticus_amd.workloads import V
+        ms_roots.append(yh[V["DISK_MASS_STELLAR"], r] + yh[V["SPHEROID_MASS_STELLAR"], r])
+        gyr = sc.node_gyr()
+        rec = {"trees": int(f.n_trees), "nodes": int(f.n_nodes), "operations": int(sc.n_ops), "gpu_seconds": gpu_s,
+               "host_seconds": host_s}
+        per.append(rec)
+        for key in ("nodes", "operations"):
+            totals[key] += rec[key]
+        totals["node_gyr"] += gyr
+        totals["gpu_seconds"] += gpu_s
+        totals["host_seconds"] += host_s
+        log(f"large-forest stream: forest {k + 1}/{n_forests}: {f.n_nodes} nodes, {sc.n_ops} operations, GPU "
+            f"{gpu_s:.2f} s, host build+walk {host_s:.2f} s")
+        sc.close()
+        del ev, y, yh, st, f, sc, y0, cur
+        if th is not None:
+            th.join()
+            cur = nxt.pop("v")
+            th = None
+    wall = time.perf_counter() - t_wall
+    ms = np.concatenate(ms_roots)
+    bins = np.arange(6.0, 13.01, 0.25)
+    smf = np.histogram(np.log10(np.maximum(ms, 1e-30)), bins)[0]
+    out = dict(totals, workload=f"{n_trees} PCH trees of the large-forest workload's roots (log-uniform "
+                                f"1e11-1e14 Msun, resolution 1e9 Msun, z=8 -> 0) as {n_forests} forests of "
+                                f"{per_forest}, forest k from builder seed k", trees=n_trees, forests=n_forests,
+               wall_seconds=wall, node_gyr_per_s_gpu=totals["node_gyr"] / totals["gpu_seconds"],
+               node_gyr_per_s_wall=totals["node_gyr"] / wall, statuses_ok=ok,
+               smf_log10_mstar_bins_0_25dex={"lower_edges": [float(b) for b in bins[:-1]],
+                                             "counts": [int(c) for c in smf]},
+               per_forest=per)
+    log(f"large-forest stream: {n_trees} trees, {totals['nodes']} nodes, {totals['operations']} operations; GPU "
+        f"{totals['gpu_seconds']:.1f} s ({out['node_gyr_per_s_gpu']:.4g} node-Gyr/s), host build+walk "
+        f"{totals['host_seconds']:.1f} s, wall {wall:.1f} s")
+    return out
+
+
 def trees_leg(physics, table, tol_abs, tol_rel, args):
     """Merger trees in the reference's walk order as operation graphs: the
     64-tree workload (roots 1e10-1e12 Msun, z=5 -> 0), a larger forest (roots
@@ -655,6 +741,9 @@ def trees_leg(physics, table, tol_abs, tol_rel, args):
                                    f"{args.large_trees} PCH trees, seed 0, roots log-uniform 1e11-1e13 Msun, "
                                    f"resolution 1e9 Msun, z=8 -> 0", threads, cpu=not args.no_cpu,
                                    split=args.tree_splits)
+    if args.c5_stream_trees > 0:
+        out["c5_stream"] = large_forest_stream(physics, table, tol_abs, tol_rel, args.c5_stream_trees,
+                                               args.c5_forest_trees)
     if args.c5_trees > 0:
         # a sample of the 4096-tree large-forest workload: its first trees
         # (the builder seeds every tree by its index), the same roots'
@@ -689,6 +778,10 @@ def main():
                          "streams) or FxS (F forests on S streams); the fastest is reported")
     ap.add_argument("--large-trees", type=int, default=256,
                     help="trees in the larger forest of the tree leg (0 skips it)")
+    ap.add_argument("--c5-stream-trees", type=int, default=0,
+                    help="trees of the large-forest workload streamed as consecutive forests (0 skips it; 4096 is "
+                         "the whole workload)")
+    ap.add_argument("--c5-forest-trees", type=int, default=64, help="trees per forest of the streamed leg")
     ap.add_argument("--c5-trees", type=int, default=16,
                     help="trees of the 4096-tree large-forest workload (roots 1e11-1e14 Msun, z=8) run as a sample "
                          "(0 skips it)")
