@@ -640,17 +640,19 @@ def surrogate_forest_route(se, physics, table, tol_abs, tol_rel, n_trees):
     return out
 
 
-def large_forest_stream(physics, table, tol_abs, tol_rel, n_trees, per_forest):
+def large_forest_stream(physics, table, tol_abs, tol_rel, n_trees, per_forest, cpu_threads=0):
     """The large-forest workload streamed: n_trees PCH trees (roots drawn as
     the 4096-tree workload's, log-uniform 1e11-1e14 Msun, resolution 1e9
     Msun, z=8 -> 0) as consecutive forests of per_forest trees (forest k from
     builder seed k).  While the GPU evolves forest k as one operation graph,
     a host thread builds and walks forest k+1 (the library calls release the
     interpreter lock).  Reports the totals, the GPU and host time per forest
-    and the z=0 stellar-mass function of the roots."""
+    and the z=0 stellar-mass function of the roots; with cpu_threads, also
+    the CPU oracle's tree runner on every forest (trees in parallel) and its
+    z=0 stellar masses against the GPU's."""
     import threading
     import torch
-    from galacticus_amd import trees as tr, workloads as wl
+    from galacticus_amd import params as gp, trees as tr, workloads as wl
     from galacticus_amd.evolver import NodeEvolverBatched
     cosmo = tr.cosmology(physics)
     t0 = float(wl.scale_factor_to_cosmic_time(1.0))
@@ -667,6 +669,10 @@ def large_forest_stream(physics, table, tol_abs, tol_rel, n_trees, per_forest):
         return f, sc, f.initial_state(cosmo, wp), time.perf_counter() - t
 
     totals = {"nodes": 0, "operations": 0, "node_gyr": 0.0, "gpu_seconds": 0.0, "host_seconds": 0.0}
+    if cpu_threads:
+        totals["cpu_seconds"] = 0.0
+    rel_all = []
+    moved_cpu = 0
     per = []
     ms_roots = []
     ok = True
@@ -675,7 +681,9 @@ def large_forest_stream(physics, table, tol_abs, tol_rel, n_trees, per_forest):
     th = None
     cur = prepare(0)
     for k in range(n_forests):
-        if k + 1 < n_forests:
+        # the next forest's walk overlaps this one's GPU run, unless the CPU
+        # runner is timed too (it would share the host's cores with the walk)
+        if k + 1 < n_forests and not cpu_threads:
             th = threading.Thread(target=lambda j=k + 1: nxt.__setitem__("v", prepare(j)))
             th.start()
         f, sc, y0, host_s = cur
@@ -690,10 +698,27 @@ def large_forest_stream(physics, table, tol_abs, tol_rel, n_trees, per_forest):
         ok = ok and bool((st.cpu().numpy() == 0).all())
         r = f.roots()
         from galacticus_amd.workloads import V
-        ms_roots.append(yh[V["DISK_MASS_STELLAR"], r] + yh[V["SPHEROID_MASS_STELLAR"], r])
+        mg = yh[V["DISK_MASS_STELLAR"], r] + yh[V["SPHEROID_MASS_STELLAR"], r]
+        ms_roots.append(mg)
         gyr = sc.node_gyr()
         rec = {"trees": int(f.n_trees), "nodes": int(f.n_nodes), "operations": int(sc.n_ops), "gpu_seconds": gpu_s,
                "host_seconds": host_s}
+        if cpu_threads:
+            import oracle
+            t = time.perf_counter()
+            yc, _, _, bad = oracle.tree_run(sc.ops, y0, -np.ones(f.n_nodes), f.tree_of_node(), f.n_trees, physics,
+                                            table, tol_abs, tol_rel, gp.FLAG_REUSE_STEP_SIZE, 0.25,
+                                            root_tol_rel=1.0e-6, nthreads=cpu_threads)
+            cpu_s = time.perf_counter() - t
+            mc = yc[V["DISK_MASS_STELLAR"], r] + yc[V["SPHEROID_MASS_STELLAR"], r]
+            rel_all.append(np.abs(mg - mc) / np.maximum(mc, 1e-300))
+            bins_c = np.arange(6.0, 13.01, 0.25)
+            moved_cpu += int(np.abs(np.histogram(np.log10(np.maximum(mg, 1e-30)), bins_c)[0] -
+                                    np.histogram(np.log10(np.maximum(mc, 1e-30)), bins_c)[0]).sum())
+            rec["cpu_seconds"] = cpu_s
+            rec["cpu_failed_evolves"] = int(bad)
+            totals["cpu_seconds"] += cpu_s
+            del yc
         per.append(rec)
         for key in ("nodes", "operations"):
             totals[key] += rec[key]
@@ -708,6 +733,8 @@ def large_forest_stream(physics, table, tol_abs, tol_rel, n_trees, per_forest):
             th.join()
             cur = nxt.pop("v")
             th = None
+        elif k + 1 < n_forests:
+            cur = prepare(k + 1)
     wall = time.perf_counter() - t_wall
     ms = np.concatenate(ms_roots)
     bins = np.arange(6.0, 13.01, 0.25)
@@ -720,6 +747,17 @@ def large_forest_stream(physics, table, tol_abs, tol_rel, n_trees, per_forest):
                smf_log10_mstar_bins_0_25dex={"lower_edges": [float(b) for b in bins[:-1]],
                                              "counts": [int(c) for c in smf]},
                per_forest=per)
+    if cpu_threads:
+        rel = np.concatenate(rel_all)
+        out["cpu_baseline"] = {"value": totals["node_gyr"] / totals["cpu_seconds"], "unit": "node-Gyr/s",
+                               "cores": cpu_threads, "kind": "port", "seconds": totals["cpu_seconds"],
+                               "sample": "every forest of the stream, oracle/oracle.c orc_tree_run (trees in "
+                                         "parallel), cooling-radius root tolerance 1e-6"}
+        out["gpu_over_cpu"] = totals["cpu_seconds"] / totals["gpu_seconds"]
+        out["z0_stellar_mass_vs_cpu"] = {"median_relative": float(np.median(rel)), "max_relative": float(rel.max()),
+                                         "smf_bins_0.25dex_count_moved": moved_cpu, "roots": int(len(rel))}
+        log(f"large-forest stream: CPU {totals['cpu_seconds']:.1f} s on {cpu_threads} threads, GPU/CPU "
+            f"{out['gpu_over_cpu']:.1f}x, root M* median rel diff {np.median(rel):.2e}, SMF moved {moved_cpu}")
     log(f"large-forest stream: {n_trees} trees, {totals['nodes']} nodes, {totals['operations']} operations; GPU "
         f"{totals['gpu_seconds']:.1f} s ({out['node_gyr_per_s_gpu']:.4g} node-Gyr/s), host build+walk "
         f"{totals['host_seconds']:.1f} s, wall {wall:.1f} s")
@@ -743,7 +781,7 @@ def trees_leg(physics, table, tol_abs, tol_rel, args):
                                    split=args.tree_splits)
     if args.c5_stream_trees > 0:
         out["c5_stream"] = large_forest_stream(physics, table, tol_abs, tol_rel, args.c5_stream_trees,
-                                               args.c5_forest_trees)
+                                               args.c5_forest_trees, 0 if args.no_cpu else threads)
     if args.c5_trees > 0:
         # a sample of the 4096-tree large-forest workload: its first trees
         # (the builder seeds every tree by its index), the same roots'
