@@ -726,7 +726,9 @@ def large_forest_stream(physics, table, tol_abs, tol_rel, n_trees, per_forest, c
         totals["gpu_seconds"] += gpu_s
         totals["host_seconds"] += host_s
         log(f"large-forest stream: forest {k + 1}/{n_forests}: {f.n_nodes} nodes, {sc.n_ops} operations, GPU "
-            f"{gpu_s:.2f} s, host build+walk {host_s:.2f} s")
+            f"{gpu_s:.2f} s, host build+walk {host_s:.2f} s"
+            + (f", CPU {rec['cpu_seconds']:.2f} s (root M* median rel diff {np.median(rel_all[-1]):.2e})"
+               if cpu_threads else ""))
         sc.close()
         del ev, y, yh, st, f, sc, y0, cur
         if th is not None:
