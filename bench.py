@@ -785,9 +785,11 @@ def trees_leg(physics, table, tol_abs, tol_rel, args):
         out["c5_stream"] = large_forest_stream(physics, table, tol_abs, tol_rel, args.c5_stream_trees,
                                                args.c5_forest_trees, 0 if args.no_cpu else threads)
     if args.c5_trees > 0:
-        # a sample of the 4096-tree large-forest workload: its first trees
-        # (the builder seeds every tree by its index), the same roots'
-        # distribution and epochs.  The whole workload's ~4e9 operations do
+        # a sample of the 4096-tree large-forest workload: the seeds and
+        # roots of its first trees (the builder seeds every tree by its
+        # index; its rate table spans the call's largest root, so the
+        # branchings differ in detail from a build of all 4096), the same
+        # epochs.  The whole workload's ~4e9 operations do
         # not fit a host schedule here; the sample's wall time on the CPU is
         # set by its largest tree, as for the reference's OpenMP over trees.
         out["c5_sample"] = _forest_run(physics, table, tol_abs, tol_rel, args.c5_trees, (11.0, 14.0), 8.0,

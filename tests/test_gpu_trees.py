@@ -381,8 +381,10 @@ METALS = ["HOT_ABUNDANCES", "HOT_OUTFLOWED_ABUNDANCES", "HOT_UNACCRETED_ABUNDANC
 def test_large_forest_sample_budgets_and_cpu_agreement(torch_cuda, cooling_table):
     """The first 8 trees of the 4096-tree large-forest workload (PCH, seed 0,
     roots log-uniform 1e11-1e14 Msun, resolution 1e9 Msun, z=8 -> 0; the
-    builder seeds each tree by its index, so these are that workload's first
-    trees) as one operation graph on the GPU:
+    builder seeds each tree by its index, so these have that workload's first
+    seeds and roots, with branchings that differ in detail because the
+    builder's rate table spans the call's largest root) as one operation
+    graph on the GPU:
       * every evolve succeeds;
       * per tree, baryons and metals close over the nodes still alive at z=0
         (never promoted away or merged): the sources are the leaves' initial
