@@ -232,12 +232,14 @@ __device__ __forceinline__ Halo halo(const double* __restrict__ P, const Derived
 // structure) can be interleaved by the scheduler.  A rate that does not
 // apply is exactly zero, which leaves the accumulated sums unchanged.
 //
-// The operators fall into two halves that share only the halo scales and
-// the state: the galaxy half (spin growth, satellite mass loss, star
-// formation, stellar feedback, bar instability, CGM accretion and outflow
-// reincorporation) and the hot-halo half (ISCO, hot-halo structure, cooling
-// function and radius, black-hole accretion, jet and winds, cooling and
-// heating, CGM outer radius, radiated energy).  rhs_part<PART> computes one
+// The operators fall into two halves that share only the halo scales, the
+// hot halo's density profile and the state: the galaxy half (spin growth,
+// satellite mass loss, star formation, stellar feedback, bar instability,
+// CGM accretion, outflow reincorporation and the CGM outer radius) and the
+// hot-halo half (ISCO, cooling function and radius, black-hole accretion,
+// jet and winds, cooling and heating, radiated energy).  The outer radius
+// needs only the density profile, and sits in the galaxy half because the
+// hot-halo half is the longer of the two.  rhs_part<PART> computes one
 // half (or both); every path forms the derivative as f_galaxy + f_halo, so a
 // wave that computes one half while another wave computes the other gets the
 // same bits as one lane computing both.
@@ -303,6 +305,23 @@ __device__ __forceinline__ void rhs_part(const double* __restrict__ P, const Der
   const double inv_tdyn_gal = H.v_vir * rcp(r_gal) * (1.0 / kMpcPerKmsToGyr);
   const double t_vir = D.t_vir_per_v2 * H.v_vir * H.v_vir;
   const double lt = log(clamp_temperature(cd, t_vir));
+
+  // ---- hot-halo structure (both halves: the galaxy half strips it) ------
+  // beta=2/3 profile, core coreRadiusOverVirialRadius*r_vir, truncated at
+  // outerRadius (hot_halo/mass_distribution/beta_profile.F90); virial
+  // (isothermal) temperature (hot_halo/temperature_profile/virial.F90).
+  const double r_core = P[GAL_P_CORE_RADIUS_OVER_VIRIAL] * H.r_vir;
+  const double inv_r_core = H.inv_r_vir * D.inv_core;
+  const double xo = r_out * inv_r_core;
+  // shape integrals S = x - atan x and N = (x^2 - ln(1 + x^2))/2, by their
+  // Taylor series below x = 0.01 where the differences cancel
+  const double uo = xo * xo;
+  const bool small_xo = xo < 0.01;
+  const double shape_m = small_xo ? xo * uo * (1.0 / 3.0 - uo * (1.0 / 5.0 - uo * (1.0 / 7.0 - uo * (1.0 / 9.0))))
+                                  : xo - atan(xo);
+  const double inv_shape = hot_ok ? rcp(shape_m) : 0.0;
+  const double rho0 = m_hot * inv_shape * (1.0 / (4.0 * kPi)) * inv_r_core * inv_r_core * inv_r_core;
+  const double rho_out = rho0 * rcp(1.0 + xo * xo);
 
   const int ops = box ? 0 : (int)P[GAL_P_OPERATORS];
   if constexpr (GAL) {
@@ -479,25 +498,58 @@ __device__ __forceinline__ void rhs_part(const double* __restrict__ P, const Der
     f[GAL_V_HOT_ANGULAR_MOMENTUM] += rj;
     f[GAL_V_HOT_ABUNDANCES] += rz;
   }
+  // ---- CGMOuterRadiusRamPressureStripping
+  //      (circumgalactic_medium/outer_radius/ram_pressure_stripping.F90:111-242)
+  // Returning outflows push the outer radius out when it lies inside the
+  // virial radius; a central's follows the virial radius; a satellite's
+  // shrinks towards the font2008 stripping radius
+  // (hot_halo/ram_pressure_stripping/Font2008.F90:115-267) on the
+  // ram-pressure acceleration time (timescale/ram_pressure_acceleration.F90:79-134).
+  // The stripping radius is the closed form of the binding condition
+  // formFactor G M(<r) rho_hot(r) / r = F for the isothermal halo
+  // M(<r) = M_vir r / r_vir: formFactor V_vir^2 rho_hot(r) = F.
+  if (ops & GAL_OPERATOR_CGM_OUTER_RADIUS) {
+    const double rv = H.r_vir;
+    const double rho_ro = hot_ok ? rho_out : 0.0;
+    const double m_return = y[GAL_V_HOT_OUTFLOWED_MASS] * P[GAL_P_REINCORPORATION_MULTIPLIER] * H.inv_t_dyn;
+    const double rho_min = D.f_baryon * H.mass * (H.inv_r_vir * H.inv_r_vir * H.inv_r_vir) * (1.0 / (4.0 * kPi));
+    const double ret_a = m_return * (1.0 / (4.0 * kPi)) * rcp(r_out * r_out * fmax(rho_ro, rho_min));
+    const double ret_b = m_return * rcp(H.mass) * rv;
+    double ret = (r_out > 0.0 && rho_ro > 0.0) ? ret_a : (m_return > 0.0 ? ret_b : 0.0);
+    // the outer radius starts at the virial radius, an exact tie in the
+    // reference; a last-place difference between the caller's value and the
+    // recomputed virial radius is judged a tie (1e-12 relative)
+    const bool at_rv = r_out >= rv * (1.0 - 1.0e-12);
+    ret = at_rv ? 0.0 : ret;
+    // stripping radius and timescale
+    const double force = np.ram_force;
+    const double bind = P[GAL_P_RAM_PRESSURE_FORM_FACTOR] * H.v_vir * H.v_vir;
+    const double x_v = rv * inv_r_core, r_small = 1.0e-6 * rv, x_s = r_small * inv_r_core;
+    const double rho_rv = (hot_ok && at_rv) ? rho0 * rcp(1.0 + x_v * x_v) : 0.0;
+    const double rho_small = (hot_ok && r_small <= r_out) ? rho0 * rcp(1.0 + x_s * x_s) : 0.0;
+    const double r_star = fmin(r_core * sqrt(fmax(bind * rho0 * rcp(force) - 1.0, 0.0)), r_out);
+    const double r_rp = bind * rho_rv - force >= 0.0 ? rv : (bind * rho_small - force <= 0.0 ? 0.0 : r_star);
+    const double sigma = r_out * rho_ro;
+    const double t_floor = np.v_host > 0.0 ? r_out * 0.1 * rcp(np.v_host) : HUGE_VAL;
+    double tau = kMpcPerKmsToGyr * fmax(sqrt(2.0 * r_out * sigma * rcp(force)), t_floor);
+    tau = (r_out <= 0.0 || sigma <= 0.0) ? H.t_dyn : tau;
+    tau = force <= 0.0 ? kDoubleMax : tau;
+    const double dr = (r_rp < r_out && y[GAL_V_HOT_ANGULAR_MOMENTUM] > 0.0) ? (r_rp - r_out) / tau : 0.0;
+    const bool strip = np.satellite && dr != 0.0 && m_hot > 0.0 && r_out > 1.0e-3 * rv;
+    const double loss = strip ? 4.0 * kPi * rho_ro * r_out * r_out * dr : 0.0;
+    const double lm = loss * inv_m_hot;
+    f[GAL_V_HOT_OUTER_RADIUS] += ret + (np.satellite ? (strip ? dr : 0.0) : H.r_vir_rate);
+    f[GAL_V_HOT_MASS] += loss;
+    f[GAL_V_HOT_ANGULAR_MOMENTUM] += y[GAL_V_HOT_ANGULAR_MOMENTUM] * lm;
+    f[GAL_V_HOT_ABUNDANCES] += y[GAL_V_HOT_ABUNDANCES] * lm;
+    f[GAL_V_HOT_STRIPPED_MASS] -= loss;
+    f[GAL_V_HOT_STRIPPED_ABUNDANCES] -= y[GAL_V_HOT_ABUNDANCES] * lm;
+  }
   }  // galaxy half
   if constexpr (!HOT) return;
-  // ---- hot-halo structure and cooling radius ----------------------------
-  // beta=2/3 profile, core coreRadiusOverVirialRadius*r_vir, truncated at
-  // outerRadius (hot_halo/mass_distribution/beta_profile.F90); virial
-  // (isothermal) temperature (hot_halo/temperature_profile/virial.F90).
-  const double r_core = P[GAL_P_CORE_RADIUS_OVER_VIRIAL] * H.r_vir;
-  const double inv_r_core = H.inv_r_vir * D.inv_core;
-  const double xo = r_out * inv_r_core;
-  // shape integrals S = x - atan x and N = (x^2 - ln(1 + x^2))/2, by their
-  // Taylor series below x = 0.01 where the differences cancel
-  const double uo = xo * xo;
-  const bool small_xo = xo < 0.01;
-  const double shape_m = small_xo ? xo * uo * (1.0 / 3.0 - uo * (1.0 / 5.0 - uo * (1.0 / 7.0 - uo * (1.0 / 9.0))))
-                                  : xo - atan(xo);
+  // ---- cooling radius ------------------------------------------------------
   const double shape_r = small_xo ? uo * uo * (1.0 / 4.0 - uo * (1.0 / 6.0 - uo * (1.0 / 8.0 - uo * (1.0 / 10.0))))
                                   : 0.5 * uo - 0.5 * log1p(uo);
-  const double inv_shape = hot_ok ? rcp(shape_m) : 0.0;
-  const double rho0 = m_hot * inv_shape * (1.0 / (4.0 * kPi)) * inv_r_core * inv_r_core * inv_r_core;
   const double r_mean = r_core * shape_r * inv_shape;
   // composition of metallicity-only abundances, fully ionized H and He
   constexpr double kNorm = 1.0 / (kHydrogenPrimordial + kHeliumPrimordial);
@@ -514,7 +566,6 @@ __device__ __forceinline__ void rhs_part(const double* __restrict__ P, const Der
   const double t_avail = H.t_dyn;
   const double K = D.cool_k_pre * t_vir * nall_per_nh * rcp(lambda_cool * nh_per_rho);
   const double inv_k = rcp(K);
-  const double rho_out = rho0 * rcp(1.0 + xo * xo);
   double r_cool = fmin(r_core * sqrt(fmax(rho0 * t_avail * inv_k - 1.0, 0.0)), r_out);
   r_cool = (K - t_avail * rho0 > 0.0) ? 0.0 : r_cool;
   r_cool = (K - t_avail * rho_out < 0.0) ? r_out : r_cool;
@@ -641,53 +692,6 @@ __device__ __forceinline__ void rhs_part(const double* __restrict__ P, const Der
     f[GAL_V_DISK_MASS_GAS] += c;
     f[GAL_V_DISK_ABUNDANCES_GAS] += cz;
     f[GAL_V_DISK_ANGULAR_MOMENTUM] += cj * (1.0 - P[GAL_P_FRACTION_LOSS_ANGULAR_MOMENTUM]);
-  }
-  // ---- CGMOuterRadiusRamPressureStripping
-  //      (circumgalactic_medium/outer_radius/ram_pressure_stripping.F90:111-242)
-  // Returning outflows push the outer radius out when it lies inside the
-  // virial radius; a central's follows the virial radius; a satellite's
-  // shrinks towards the font2008 stripping radius
-  // (hot_halo/ram_pressure_stripping/Font2008.F90:115-267) on the
-  // ram-pressure acceleration time (timescale/ram_pressure_acceleration.F90:79-134).
-  // The stripping radius is the closed form of the binding condition
-  // formFactor G M(<r) rho_hot(r) / r = F for the isothermal halo
-  // M(<r) = M_vir r / r_vir: formFactor V_vir^2 rho_hot(r) = F.
-  if (ops & GAL_OPERATOR_CGM_OUTER_RADIUS) {
-    const double rv = H.r_vir;
-    const double rho_ro = hot_ok ? rho_out : 0.0;
-    const double m_return = y[GAL_V_HOT_OUTFLOWED_MASS] * P[GAL_P_REINCORPORATION_MULTIPLIER] * H.inv_t_dyn;
-    const double rho_min = D.f_baryon * H.mass * (H.inv_r_vir * H.inv_r_vir * H.inv_r_vir) * (1.0 / (4.0 * kPi));
-    const double ret_a = m_return * (1.0 / (4.0 * kPi)) * rcp(r_out * r_out * fmax(rho_ro, rho_min));
-    const double ret_b = m_return * rcp(H.mass) * rv;
-    double ret = (r_out > 0.0 && rho_ro > 0.0) ? ret_a : (m_return > 0.0 ? ret_b : 0.0);
-    // the outer radius starts at the virial radius, an exact tie in the
-    // reference; a last-place difference between the caller's value and the
-    // recomputed virial radius is judged a tie (1e-12 relative)
-    const bool at_rv = r_out >= rv * (1.0 - 1.0e-12);
-    ret = at_rv ? 0.0 : ret;
-    // stripping radius and timescale
-    const double force = np.ram_force;
-    const double bind = P[GAL_P_RAM_PRESSURE_FORM_FACTOR] * H.v_vir * H.v_vir;
-    const double x_v = rv * inv_r_core, r_small = 1.0e-6 * rv, x_s = r_small * inv_r_core;
-    const double rho_rv = (hot_ok && at_rv) ? rho0 * rcp(1.0 + x_v * x_v) : 0.0;
-    const double rho_small = (hot_ok && r_small <= r_out) ? rho0 * rcp(1.0 + x_s * x_s) : 0.0;
-    const double r_star = fmin(r_core * sqrt(fmax(bind * rho0 * rcp(force) - 1.0, 0.0)), r_out);
-    const double r_rp = bind * rho_rv - force >= 0.0 ? rv : (bind * rho_small - force <= 0.0 ? 0.0 : r_star);
-    const double sigma = r_out * rho_ro;
-    const double t_floor = np.v_host > 0.0 ? r_out * 0.1 * rcp(np.v_host) : HUGE_VAL;
-    double tau = kMpcPerKmsToGyr * fmax(sqrt(2.0 * r_out * sigma * rcp(force)), t_floor);
-    tau = (r_out <= 0.0 || sigma <= 0.0) ? H.t_dyn : tau;
-    tau = force <= 0.0 ? kDoubleMax : tau;
-    const double dr = (r_rp < r_out && y[GAL_V_HOT_ANGULAR_MOMENTUM] > 0.0) ? (r_rp - r_out) / tau : 0.0;
-    const bool strip = np.satellite && dr != 0.0 && m_hot > 0.0 && r_out > 1.0e-3 * rv;
-    const double loss = strip ? 4.0 * kPi * rho_ro * r_out * r_out * dr : 0.0;
-    const double lm = loss * inv_m_hot;
-    f[GAL_V_HOT_OUTER_RADIUS] += ret + (np.satellite ? (strip ? dr : 0.0) : H.r_vir_rate);
-    f[GAL_V_HOT_MASS] += loss;
-    f[GAL_V_HOT_ANGULAR_MOMENTUM] += y[GAL_V_HOT_ANGULAR_MOMENTUM] * lm;
-    f[GAL_V_HOT_ABUNDANCES] += y[GAL_V_HOT_ABUNDANCES] * lm;
-    f[GAL_V_HOT_STRIPPED_MASS] -= loss;
-    f[GAL_V_HOT_STRIPPED_ABUNDANCES] -= y[GAL_V_HOT_ABUNDANCES] * lm;
   }
   // ---- coolingEnergyRadiated (cooling_energy_radiated.F90:200-260) -------
   {
