@@ -808,7 +808,7 @@ def main():
     ap.add_argument("--seed", type=int, default=0)
     ap.add_argument("--streams", default="auto",
                     help="independent batches alternate over this many streams and evolver handles, or 'auto' "
-                         "for one per timed step (up to 32)")
+                         "for one per timed step (up to 20)")
     ap.add_argument("--launch", default="auto",
                     help="launch shape 'nodes_per_workgroup,workgroups_per_cu[,rhs_waves]', or 'auto' to time them")
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
@@ -867,10 +867,12 @@ def main():
     batch = wl.isolated_halos(args.nodes, seed=args.seed, params=physics)
     n = batch.n
     node_gyr = batch.node_gyr
-    # Every timed batch in flight at once (up to 32): each batch's tail -- the
+    # Every timed batch in flight at once (up to 20): each batch's tail -- the
     # few stiffest nodes, whose chains set the launch time -- then overlaps
-    # the others instead of queueing behind a batch on the same stream.
-    n_streams = min(max(1, args.steps), 32) if args.streams == "auto" else max(1, int(args.streams))
+    # the others instead of queueing behind a batch on the same stream.  More
+    # streams than about 24 hardware queues time-slice: 40 batches on 32
+    # streams took 21-25 ms per batch against 11-13 ms on 20.
+    n_streams = min(max(1, args.steps), 20) if args.streams == "auto" else max(1, int(args.streams))
     evs = [NodeEvolverBatched(n, physics, table, odeToleranceAbsolute=tol_abs, odeToleranceRelative=tol_rel,
                               reuseODEStepSize=True) for _ in range(n_streams)]
     ev = evs[0]
