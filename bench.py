@@ -809,6 +809,9 @@ def main():
     ap.add_argument("--streams", default="auto",
                     help="independent batches alternate over this many streams and evolver handles, or 'auto' "
                          "for one per timed step (up to 20)")
+    ap.add_argument("--no-plan", action="store_true",
+                    help="order each overlapped batch's claims inside its own evolve call instead of planning all "
+                         "batches before the evolves launch")
     ap.add_argument("--launch", default="auto",
                     help="launch shape 'nodes_per_workgroup,workgroups_per_cu[,rhs_waves]', or 'auto' to time them")
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
@@ -892,6 +895,39 @@ def main():
     # share the same queues.
     streams = stream_pool(n_streams)
     tuning = {}
+    plan_ahead = not args.no_plan
+
+    def overlapped_round(items, before=None, after=None):
+        """One batch per stream, all in flight together: items are
+        (stream index, y, h_step, scale, status, counters).  With plan_ahead
+        every batch's claim order is computed (gal_ode_batch_plan_d) before
+        any persistent evolve launches, so that no batch's ordering kernels
+        wait for CUs that other batches' evolves hold; the evolves then wait
+        for all the plans.  before/after(k) record events around item k."""
+        if plan_ahead:
+            done = []
+            for k, (j, y, h, sc, st, cnt) in enumerate(items):
+                if before:
+                    before(k)
+                evs[j].plan_device(y, npar, t0, t1, scale=sc, stream=streams[j])
+                e = torch.cuda.Event()
+                e.record(streams[j])
+                done.append(e)
+            for k, (j, y, h, sc, st, cnt) in enumerate(items):
+                for e in done:
+                    streams[j].wait_event(e)
+                evs[j].evolve_device(y, npar, t0, t1, h, scale=sc, status=st, counters=cnt, stream=streams[j],
+                                     planned=True)
+                if after:
+                    after(k)
+        else:
+            for k, (j, y, h, sc, st, cnt) in enumerate(items):
+                if before:
+                    before(k)
+                evs[j].evolve_device(y, npar, t0, t1, h, scale=sc, status=st, counters=cnt, stream=streams[j])
+                if after:
+                    after(k)
+
     shapes = ((64, 1, 1), (64, 2, 1), (64, 4, 1), (64, 1, 2), (64, 2, 2))
     if args.launch != "auto":
         v = tuple(int(x) for x in args.launch.split(","))
@@ -913,8 +949,7 @@ def main():
             ht = [h0.clone() for _ in range(n_streams)]
             torch.cuda.synchronize()
             t_a = time.perf_counter()
-            for j in range(n_streams):
-                evs[j].evolve_device(yt[j], npar, t0, t1, ht[j], stream=streams[j])
+            overlapped_round([(j, yt[j], ht[j], None, None, None) for j in range(n_streams)])
             torch.cuda.synchronize()
             ms = (time.perf_counter() - t_a) * 1e3 / n_streams
             tuning[f"{shape[0]},{shape[1]},{shape[2]}"] = ms
@@ -969,12 +1004,13 @@ def main():
     ends = [torch.cuda.Event(enable_timing=True) for _ in range(args.steps)]
     torch.cuda.synchronize()
     t_wall = time.perf_counter()
-    for k in range(args.steps):
-        i = args.warmup + k
-        j = k % n_streams
-        starts[k].record(streams[j])
-        launch_step(i, k)
-        ends[k].record(streams[j])
+    # rounds of one batch per stream (one round when steps <= streams)
+    for r0 in range(0, args.steps, n_streams):
+        ks = list(range(r0, min(args.steps, r0 + n_streams)))
+        overlapped_round([(k % n_streams, ys[args.warmup + k], hs[args.warmup + k], scales[k % n_streams],
+                           statuses[k % n_streams], counters[k % n_streams]) for k in ks],
+                         before=lambda m, ks=ks: starts[ks[m]].record(streams[ks[m] % n_streams]),
+                         after=lambda m, ks=ks: ends[ks[m]].record(streams[ks[m] % n_streams]))
     torch.cuda.synchronize()
     wall = time.perf_counter() - t_wall
     kernel_ms = float(np.mean([s.elapsed_time(e) for s, e in zip(starts, ends)]))
@@ -1018,6 +1054,8 @@ def main():
                                                                       if launch[2] == 2 else "")),
             "launch": {"nodes_per_workgroup": launch[0], "workgroups_per_cu": launch[1], "rhs_waves": launch[2],
                        "streams": n_streams,
+                       "claim_order": ("planned for every batch of a round before its evolves launch" if plan_ahead
+                                       else "inside each evolve call"),
                        "tuning_ms_per_step": tuning},
         },
         "single_launch": {

@@ -19,7 +19,7 @@ ERRORS = {-1: "GAL_E_INVALID", -2: "GAL_E_HIP", -3: "GAL_E_NOMEM", -4: "GAL_E_NO
 EXPORTS = [
     "gal_ode_version", "gal_ode_device_check", "gal_ode_batch_create", "gal_ode_batch_destroy",
     "gal_ode_batch_set_physics", "gal_ode_batch_set_cooling_table", "gal_ode_batch_set_launch",
-    "gal_ode_batch_evolve_d", "gal_ode_batch_evolve_h", "gal_ode_rhs_d", "gal_ode_hadjust_d",
+    "gal_ode_batch_evolve_d", "gal_ode_batch_plan_d", "gal_ode_batch_evolve_planned_d", "gal_ode_batch_evolve_h", "gal_ode_rhs_d", "gal_ode_hadjust_d",
     "gal_ode_batch_autotune", "gal_ode_evolve_static_d", "gal_ode_batch_set_rhs_waves", "gal_ode_batch_get_launch",
     "gal_ode_batch_set_wall_limit", "gal_ode_batch_set_graph_stall_limit",
 ]
@@ -59,6 +59,8 @@ def load():
     L.gal_ode_batch_set_cooling_table.argtypes = [vp, dp, c_int, c_double, c_double, c_int, c_double, c_double]
     L.gal_ode_batch_set_launch.argtypes = [vp, c_int, c_int]
     L.gal_ode_batch_evolve_d.argtypes = [vp, c_int, c_int, vp, vp, vp, vp, vp, vp, vp, vp, vp]
+    L.gal_ode_batch_plan_d.argtypes = [vp, c_int, c_int, vp, vp, vp, vp, vp, vp]
+    L.gal_ode_batch_evolve_planned_d.argtypes = [vp, c_int, c_int, vp, vp, vp, vp, vp, vp, vp, vp, vp]
     L.gal_ode_batch_evolve_h.argtypes = [vp, c_int, dp, dp, dp, dp, dp, dp, ip, i64p]
     L.gal_ode_rhs_d.argtypes = [vp, c_int, c_int, vp, vp, vp, vp, vp]
     L.gal_ode_hadjust_d.argtypes = [c_int, c_int, c_int, c_double, c_double, c_double, c_double, vp, vp, vp, vp,

@@ -33,6 +33,10 @@ struct gal_ode_batch {
   double* d_kwork = nullptr;  // lane-private stage derivatives for four workgroups per CU
   int kwork_lanes = 0;
   int* d_order = nullptr;
+  // claim order planned by gal_ode_batch_plan_d in d_order, for the next
+  // gal_ode_batch_evolve_planned_d of the same y, n and ld
+  const double* plan_y = nullptr;
+  int plan_n = 0, plan_ld = 0;
   int* d_index = nullptr;
   float* d_key = nullptr;
   float* d_key_sorted = nullptr;

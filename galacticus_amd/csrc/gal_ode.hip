@@ -386,7 +386,10 @@ __global__ __launch_bounds__(BLOCK * WAVES, 1) void evolve_kernel(EvolveArgs a) 
             }
           }
         } else {
-          const int idx = wave_claim(want, a.work, n_claim, a.order);
+          // an order entry outside the batch (a plan the caller did not let
+          // finish) ends this lane's claims instead of addressing past it
+          const int claimed = wave_claim(want, a.work, n_claim, a.order);
+          const int idx = claimed < a.n ? claimed : -1;
           if (want) {
             if (idx < 0) {
               want = false;
@@ -1405,6 +1408,24 @@ extern "C" int gal_ode_evolve_static_d(const double* params, int n_params, const
   return GAL_OK;
 }
 
+// Claim order of an evolve: every node's cost estimate, sorted so that the
+// costliest nodes are claimed first (their chains then overlap the bulk).
+// Writes h->d_order on stream s.
+static int plan_claim_order(gal_ode_batch* h, int n_subset, const int* subset, int ld, const double* y,
+                            const double* node_par, const double* t_start, const double* t_end, const double* scale,
+                            const int* n_subset_dev, hipStream_t s) {
+  const int threads = 256;
+  hipLaunchKernelGGL(cost_probe_kernel, dim3((n_subset + threads - 1) / threads), dim3(threads), 0, s, n_subset,
+                     subset, ld, y, node_par, t_start, t_end, scale, (const double*)h->d_params,
+                     (const double*)h->d_table, h->D, h->cd, (int)h->params[GAL_P_PHYSICS_MODE],
+                     (h->flags & GAL_FLAG_SCALES_PROVIDED) ? 1 : 0, n_subset_dev, h->d_key, h->d_index);
+  HIP_TRY(hipGetLastError());
+  size_t tb = h->sort_temp_bytes;
+  HIP_TRY(hipcub::DeviceRadixSort::SortPairsDescending(h->d_sort_temp, tb, h->d_key, h->d_key_sorted, h->d_index,
+                                                       h->d_order, n_subset, 0, 32, s));
+  return GAL_OK;
+}
+
 int gal_ode_evolve_subset(gal_ode_batch* h, int n, int ld, double* y, const double* node_par, const double* t_start,
                           const double* t_end, double* h_step, double* scale, int* status, int64_t* counters,
                           const int* subset, int n_subset, hipStream_t s, const int* n_subset_dev) {
@@ -1416,19 +1437,45 @@ int gal_ode_evolve_subset(gal_ode_batch* h, int n, int ld, double* y, const doub
   a.n_claim = n_subset;
   a.n_claim_dev = n_subset_dev;   // when set, n_subset is the list's capacity
   a.order = subset;
+  h->plan_y = nullptr;            // an evolve of its own replaces any pending plan
   if (h->flags & GAL_FLAG_COST_ORDER) {
-    const int threads = 256;
-    hipLaunchKernelGGL(cost_probe_kernel, dim3((n_subset + threads - 1) / threads), dim3(threads), 0, s, n_subset,
-                       subset, ld, (const double*)y, node_par, t_start, t_end, (const double*)scale,
-                       (const double*)h->d_params, (const double*)h->d_table, h->D, h->cd, a.mode,
-                       (h->flags & GAL_FLAG_SCALES_PROVIDED) ? 1 : 0, n_subset_dev, h->d_key, h->d_index);
-    HIP_TRY(hipGetLastError());
-    size_t tb = h->sort_temp_bytes;
-    HIP_TRY(hipcub::DeviceRadixSort::SortPairsDescending(h->d_sort_temp, tb, h->d_key, h->d_key_sorted, h->d_index,
-                                                         h->d_order, n_subset, 0, 32, s));
+    rc = plan_claim_order(h, n_subset, subset, ld, y, node_par, t_start, t_end, scale, n_subset_dev, s);
+    if (rc != GAL_OK) return rc;
     a.order = h->d_order;
   }
   return launch_evolve(h, a, s);
+}
+
+int gal_ode_batch_plan_d(gal_ode_batch* h, int n, int ld, const double* y, const double* node_par,
+                         const double* t_start, const double* t_end, const double* scale, void* stream) {
+  if (!h || n < 0 || n > h->n_max || ld < n) return GAL_E_INVALID;
+  if (!h->have_physics || !h->have_table || !(h->flags & GAL_FLAG_COST_ORDER)) return GAL_E_INVALID;
+  if (n > 0 && (!y || !node_par || !t_start || !t_end || ((h->flags & GAL_FLAG_SCALES_PROVIDED) && !scale)))
+    return GAL_E_INVALID;
+  h->plan_y = nullptr;
+  if (n > 0) {
+    const int rc = plan_claim_order(h, n, nullptr, ld, y, node_par, t_start, t_end, scale, nullptr, (hipStream_t)stream);
+    if (rc != GAL_OK) return rc;
+  }
+  h->plan_y = y;
+  h->plan_n = n;
+  h->plan_ld = ld;
+  return GAL_OK;
+}
+
+int gal_ode_batch_evolve_planned_d(gal_ode_batch* h, int n, int ld, double* y, const double* node_par,
+                                   const double* t_start, const double* t_end, double* h_step, double* scale,
+                                   int* status, int64_t* counters, void* stream) {
+  if (!h || !y || h->plan_y != y || h->plan_n != n || h->plan_ld != ld) return GAL_E_INVALID;
+  h->plan_y = nullptr;   // a plan serves one evolve
+  EvolveArgs a;
+  const int rc = make_args(h, n, ld, y, node_par, t_start, t_end, h_step, scale, status, counters, a);
+  if (rc != GAL_OK) return rc;
+  if (n == 0) return GAL_OK;
+  a.n_claim = n;
+  a.n_claim_dev = nullptr;
+  a.order = h->d_order;
+  return launch_evolve(h, a, (hipStream_t)stream);
 }
 
 int gal_ode_evolve_graph(gal_ode_batch* h, int ld, double* y, double* h_step, double* scale, int* status,

@@ -145,8 +145,11 @@ class NodeEvolverBatched:
                 raise ValueError(f"per-node arrays need at least {n} entries")
 
     def evolve_device(self, y, node_par, t_start, t_end, h_step, scale=None, status=None, counters=None,
-                      stream=None):
-        """Evolve in place on device tensors (float64 SoA, y: (30, ld))."""
+                      stream=None, planned=False):
+        """Evolve in place on device tensors (float64 SoA, y: (30, ld)).
+
+        planned=True launches the evolve in the claim order that plan_device
+        computed for the same y (gal_ode_batch_evolve_planned_d)."""
         import torch
         n = int(t_start.shape[0])
         ld = int(y.shape[1])
@@ -158,10 +161,21 @@ class NodeEvolverBatched:
         if scale is None:
             scale = torch.empty_like(y)
         s = ctypes.c_void_p(0 if stream is None else stream.cuda_stream)
-        _lib.check(self.L.gal_ode_batch_evolve_d(self.handle, n, ld, _ptr(y), _ptr(node_par), _ptr(t_start),
-                                                 _ptr(t_end), _ptr(h_step), _ptr(scale), _ptr(status),
-                                                 _ptr(counters), s), "gal_ode_batch_evolve_d")
+        fn = self.L.gal_ode_batch_evolve_planned_d if planned else self.L.gal_ode_batch_evolve_d
+        _lib.check(fn(self.handle, n, ld, _ptr(y), _ptr(node_par), _ptr(t_start), _ptr(t_end), _ptr(h_step),
+                      _ptr(scale), _ptr(status), _ptr(counters), s),
+                   "gal_ode_batch_evolve_planned_d" if planned else "gal_ode_batch_evolve_d")
         return status
+
+    def plan_device(self, y, node_par, t_start, t_end, scale=None, stream=None):
+        """Order the nodes of the next evolve_device(..., planned=True) of the
+        same y for claiming (gal_ode_batch_plan_d), without launching it."""
+        n = int(t_start.shape[0])
+        ld = int(y.shape[1])
+        self._check_shapes(n, y, node_par, t_end, scale=scale)
+        s = ctypes.c_void_p(0 if stream is None else stream.cuda_stream)
+        _lib.check(self.L.gal_ode_batch_plan_d(self.handle, n, ld, _ptr(y), _ptr(node_par), _ptr(t_start),
+                                               _ptr(t_end), _ptr(scale), s), "gal_ode_batch_plan_d")
 
     def evolve(self, y, node_par, t_start, t_end, h_step, scale=None):
         """Host arrays in, host arrays out (staged through HBM by the library).

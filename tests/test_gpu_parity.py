@@ -264,3 +264,54 @@ def test_evolve_captured_in_a_hip_graph(torch_cuda, cooling_table):
         ev.evolve_device(y_e, npar, t0, t1, h_e)
         torch.cuda.synchronize()
         assert torch.equal(y_e, y_ref)
+
+
+def test_planned_evolve_equals_evolve(torch_cuda, cooling_table):
+    """gal_ode_batch_plan_d then gal_ode_batch_evolve_planned_d (the claim
+    order computed ahead of the launch) gives the same states, step sizes,
+    statuses and counters as gal_ode_batch_evolve_d, bit for bit, also with
+    several handles planned on their streams before any evolve; a plan serves
+    one evolve, needs the same y, n and ld, and another evolve discards it."""
+    torch = torch_cuda
+    from galacticus_amd import _lib
+    physics = gp.physics_params()
+    b = wl.isolated_halos(3000, seed=33, params=physics)
+    d = lambda a: torch.from_numpy(np.ascontiguousarray(a)).cuda()
+    npar, t0, t1 = d(b.node_par), d(b.t_start), d(b.t_end)
+    ref = _evolver(b.n, physics)
+    y_r, h_r = d(b.y), d(b.h_step)
+    c_r = torch.zeros(3 * b.n, dtype=torch.int64, device="cuda")
+    st_r = ref.evolve_device(y_r, npar, t0, t1, h_r, counters=c_r)
+    torch.cuda.synchronize()
+    assert (st_r.cpu() == 0).all()
+    evs = [_evolver(b.n, physics) for _ in range(3)]
+    streams = [torch.cuda.Stream() for _ in evs]
+    ys = [d(b.y) for _ in evs]
+    hs = [d(b.h_step) for _ in evs]
+    cs = [torch.zeros(3 * b.n, dtype=torch.int64, device="cuda") for _ in evs]
+    sts = [torch.full((b.n,), -7, dtype=torch.int32, device="cuda") for _ in evs]
+    torch.cuda.synchronize()
+    for ev, y, s in zip(evs, ys, streams):
+        print("plan", flush=True)
+        ev.plan_device(y, npar, t0, t1, stream=s)
+    for ev, y, h, c, st, s in zip(evs, ys, hs, cs, sts, streams):
+        print("planned evolve", flush=True)
+        ev.evolve_device(y, npar, t0, t1, h, status=st, counters=c, stream=s, planned=True)
+    torch.cuda.synchronize()
+    for y, h, c, st in zip(ys, hs, cs, sts):
+        assert torch.equal(y, y_r) and torch.equal(h, h_r) and torch.equal(c, c_r) and torch.equal(st, st_r)
+    ev = evs[0]
+    y = d(b.y)
+    # no plan left: the evolve above consumed it
+    with pytest.raises(_lib.GalError):
+        ev.evolve_device(y, npar, t0, t1, d(b.h_step), planned=True)
+    # a plan for other inputs does not serve this y
+    ev.plan_device(d(b.y), npar, t0, t1)
+    with pytest.raises(_lib.GalError):
+        ev.evolve_device(y, npar, t0, t1, d(b.h_step), planned=True)
+    # an evolve of its own discards the plan
+    ev.plan_device(y, npar, t0, t1)
+    ev.evolve_device(d(b.y), npar, t0, t1, d(b.h_step))
+    with pytest.raises(_lib.GalError):
+        ev.evolve_device(y, npar, t0, t1, d(b.h_step), planned=True)
+    torch.cuda.synchronize()
