@@ -640,17 +640,19 @@ def surrogate_forest_route(se, physics, table, tol_abs, tol_rel, n_trees):
     return out
 
 
-def large_forest_stream(physics, table, tol_abs, tol_rel, n_trees, per_forest, cpu_threads=0):
+def large_forest_stream(physics, table, tol_abs, tol_rel, n_trees, per_forest, cpu_threads=0, walkers=3):
     """The large-forest workload streamed: n_trees PCH trees (roots drawn as
     the 4096-tree workload's, log-uniform 1e11-1e14 Msun, resolution 1e9
     Msun, z=8 -> 0) as consecutive forests of per_forest trees (forest k from
     builder seed k).  While the GPU evolves forest k as one operation graph,
-    a host thread builds and walks forest k+1 (the library calls release the
-    interpreter lock).  Reports the totals, the GPU and host time per forest
+    `walkers` host threads build and walk forests k+1 .. k+walkers (the
+    library calls release the interpreter lock; each walk spreads its trees
+    over the host's cores, and concurrent walks overlap one forest's largest
+    trees and page faults with the others).  Reports the totals, the GPU and host time per forest
     and the z=0 stellar-mass function of the roots; with cpu_threads, also
     the CPU oracle's tree runner on every forest (trees in parallel) and its
     z=0 stellar masses against the GPU's."""
-    import threading
+    from concurrent.futures import ThreadPoolExecutor
     import torch
     from galacticus_amd import params as gp, trees as tr, workloads as wl
     from galacticus_amd.evolver import NodeEvolverBatched
@@ -666,7 +668,7 @@ def large_forest_stream(physics, table, tol_abs, tol_rel, n_trees, per_forest, c
         f = tr.Forest.build_pch(min(per_forest, n_trees - k * per_forest), roots[k * per_forest:(k + 1) * per_forest],
                                 t0, cosmo, tr.pch_params(1e9, t8), seed=k)
         sc = tr.Schedule(f, cosmo, wp)
-        return f, sc, f.initial_state(cosmo, wp), time.perf_counter() - t
+        return f, sc, f.initial_state(cosmo, wp), sc.node_gyr(), time.perf_counter() - t
 
     totals = {"nodes": 0, "operations": 0, "node_gyr": 0.0, "gpu_seconds": 0.0, "host_seconds": 0.0}
     if cpu_threads:
@@ -677,16 +679,18 @@ def large_forest_stream(physics, table, tol_abs, tol_rel, n_trees, per_forest, c
     ms_roots = []
     ok = True
     t_wall = time.perf_counter()
-    nxt = {}
-    th = None
+    # the next forests' walks overlap this one's GPU run, unless the CPU
+    # runner is timed too (it would share the host's cores with the walks)
+    ahead = 0 if cpu_threads else max(1, int(walkers))
+    pool = ThreadPoolExecutor(max_workers=ahead) if ahead else None
+    futs = {}
     cur = prepare(0)
     for k in range(n_forests):
-        # the next forest's walk overlaps this one's GPU run, unless the CPU
-        # runner is timed too (it would share the host's cores with the walk)
-        if k + 1 < n_forests and not cpu_threads:
-            th = threading.Thread(target=lambda j=k + 1: nxt.__setitem__("v", prepare(j)))
-            th.start()
-        f, sc, y0, host_s = cur
+        if pool:
+            for j in range(k + 1, min(n_forests, k + 1 + ahead)):
+                if j not in futs:
+                    futs[j] = pool.submit(prepare, j)
+        f, sc, y0, gyr, host_s = cur
         ev = NodeEvolverBatched(f.n_nodes, physics, table, odeToleranceAbsolute=tol_abs, odeToleranceRelative=tol_rel)
         y = torch.from_numpy(y0).cuda()
         torch.cuda.synchronize()
@@ -700,7 +704,6 @@ def large_forest_stream(physics, table, tol_abs, tol_rel, n_trees, per_forest, c
         from galacticus_amd.workloads import V
         mg = yh[V["DISK_MASS_STELLAR"], r] + yh[V["SPHEROID_MASS_STELLAR"], r]
         ms_roots.append(mg)
-        gyr = sc.node_gyr()
         rec = {"trees": int(f.n_trees), "nodes": int(f.n_nodes), "operations": int(sc.n_ops), "gpu_seconds": gpu_s,
                "host_seconds": host_s}
         if cpu_threads:
@@ -731,12 +734,10 @@ def large_forest_stream(physics, table, tol_abs, tol_rel, n_trees, per_forest, c
                if cpu_threads else ""))
         sc.close()
         del ev, y, yh, st, f, sc, y0, cur
-        if th is not None:
-            th.join()
-            cur = nxt.pop("v")
-            th = None
-        elif k + 1 < n_forests:
-            cur = prepare(k + 1)
+        if k + 1 < n_forests:
+            cur = futs.pop(k + 1).result() if pool else prepare(k + 1)
+    if pool:
+        pool.shutdown()
     wall = time.perf_counter() - t_wall
     ms = np.concatenate(ms_roots)
     bins = np.arange(6.0, 13.01, 0.25)
@@ -783,7 +784,8 @@ def trees_leg(physics, table, tol_abs, tol_rel, args):
                                    split=args.tree_splits)
     if args.c5_stream_trees > 0:
         out["c5_stream"] = large_forest_stream(physics, table, tol_abs, tol_rel, args.c5_stream_trees,
-                                               args.c5_forest_trees, 0 if args.no_cpu else threads)
+                                               args.c5_forest_trees, 0 if args.no_cpu else threads,
+                                               walkers=args.c5_walkers)
     if args.c5_trees > 0:
         # a sample of the 4096-tree large-forest workload: the seeds and
         # roots of its first trees (the builder seeds every tree by its
@@ -827,6 +829,8 @@ def main():
                     help="trees of the large-forest workload streamed as consecutive forests (0 skips it; 4096 is "
                          "the whole workload)")
     ap.add_argument("--c5-forest-trees", type=int, default=64, help="trees per forest of the streamed leg")
+    ap.add_argument("--c5-walkers", type=int, default=3,
+                    help="forests of the streamed leg built and walked on the host at once, ahead of the GPU")
     ap.add_argument("--c5-trees", type=int, default=16,
                     help="trees of the 4096-tree large-forest workload (roots 1e11-1e14 Msun, z=8) run as a sample "
                          "(0 skips it)")

@@ -896,6 +896,15 @@ int gal_schedule_ops(const gal_schedule* s, gal_schedule_op* ops) {
   return GAL_OK;
 }
 
+int gal_schedule_node_gyr(const gal_schedule* s, double* node_gyr) {
+  if (!s || !node_gyr) return GAL_E_INVALID;
+  double sum = 0.0;
+  for (const gal_schedule_op& o : s->ops)
+    if (o.kind == GAL_OP_EVOLVE) sum += o.t1 - o.t0;
+  *node_gyr = sum;
+  return GAL_OK;
+}
+
 int gal_schedule_destroy(gal_schedule* s) {
   delete s;
   return GAL_OK;

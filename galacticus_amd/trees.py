@@ -125,6 +125,7 @@ def _bind(L):
     L.gal_schedule_build.argtypes = [vp, ctypes.POINTER(Cosmology), ctypes.POINTER(WalkParams), ctypes.POINTER(vp)]
     L.gal_schedule_size.argtypes = [vp, ctypes.POINTER(i64), ctypes.POINTER(c_int)]
     L.gal_schedule_ops.argtypes = [vp, vp]
+    L.gal_schedule_node_gyr.argtypes = [vp, dp]
     L.gal_schedule_destroy.argtypes = [vp]
     L.gal_forest_initial_state.argtypes = [vp, ctypes.POINTER(Cosmology), ctypes.POINTER(WalkParams),
                                            ctypes.c_double, ctypes.c_double, c_int, dp]
@@ -137,7 +138,8 @@ def _bind(L):
                                            ctypes.c_int64, ctypes.POINTER(ctypes.c_int64), vp]
     for name in ("gal_forest_build_pch", "gal_forest_from_arrays", "gal_forest_size", "gal_forest_arrays",
                  "gal_forest_destroy", "gal_cosmology_sigma", "gal_cosmology_delta_critical", "gal_schedule_build",
-                 "gal_schedule_size", "gal_schedule_ops", "gal_schedule_destroy", "gal_forest_initial_state",
+                 "gal_schedule_size", "gal_schedule_ops", "gal_schedule_node_gyr", "gal_schedule_destroy",
+                 "gal_forest_initial_state",
                  "gal_tree_evolve_d", "gal_tree_evolve_waves_d", "gal_tree_evolve_surrogate_d", "gal_tree_evolve_record_d",
                  "gal_node_events_h"):
         getattr(L, name).restype = c_int
@@ -318,13 +320,28 @@ class Schedule:
         n, nw = ctypes.c_int64(), ctypes.c_int()
         self.L.gal_schedule_size(h, ctypes.byref(n), ctypes.byref(nw))
         self.n_ops, self.n_waves = int(n.value), int(nw.value)
-        self.ops = np.zeros(self.n_ops, dtype=OP_DTYPE)
-        if self.n_ops:
-            self.L.gal_schedule_ops(h, ctypes.c_void_p(self.ops.ctypes.data))
+        self._ops = None
+
+    @property
+    def ops(self):
+        """The operations as a structured array (OP_DTYPE), copied out of the
+        library on first use (88 B per operation: a large forest's schedule
+        is run without it)."""
+        if self._ops is None:
+            if not getattr(self, "handle", None):
+                raise ValueError("schedule closed")
+            ops = np.zeros(self.n_ops, dtype=OP_DTYPE)
+            if self.n_ops:
+                _lib.check(self.L.gal_schedule_ops(self.handle, ctypes.c_void_p(ops.ctypes.data)),
+                           "gal_schedule_ops")
+            self._ops = ops
+        return self._ops
 
     def node_gyr(self):
-        ev = self.ops[self.ops["kind"] == OP_EVOLVE]
-        return float(np.sum(ev["t1"] - ev["t0"]))
+        """Node-Gyr the schedule evolves: its evolve intervals' lengths summed."""
+        v = ctypes.c_double()
+        _lib.check(self.L.gal_schedule_node_gyr(self.handle, ctypes.byref(v)), "gal_schedule_node_gyr")
+        return float(v.value)
 
     def close(self):
         if getattr(self, "handle", None):

@@ -100,6 +100,10 @@ int gal_schedule_build(const gal_forest* f, const gal_cosmology* cosmo, const ga
                        gal_schedule** out);
 int gal_schedule_size(const gal_schedule* s, int64_t* n_ops, int* n_waves);
 int gal_schedule_ops(const gal_schedule* s, gal_schedule_op* ops);
+/* Node-Gyr the schedule evolves: the lengths t1 - t0 of its EVOLVE
+   operations summed in schedule order, without copying the operations out.
+   No counterpart in the reference (a workload statistic). */
+int gal_schedule_node_gyr(const gal_schedule* s, double* node_gyr);
 int gal_schedule_destroy(gal_schedule* s);
 
 /* Initial galaxies of the leaf nodes (state y[v*ld + i], SoA, for every node

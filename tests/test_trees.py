@@ -142,6 +142,25 @@ def _steps(t, t_end, cos):
     return out
 
 
+def test_schedule_node_gyr_and_lazy_operations(cosmo):
+    """gal_schedule_node_gyr sums the EVOLVE intervals without copying the
+    operations; the operations are copied out on first use only, and a
+    closed schedule refuses both."""
+    f = tr.Forest.build_pch(6, np.geomspace(1e11, 1e12, 6), T0, cosmo, tr.pch_params(1e10, T_Z5), seed=3)
+    s = tr.Schedule(f, cosmo, tr.walk_params(T0))
+    assert s._ops is None
+    g = s.node_gyr()
+    ev = s.ops[s.ops["kind"] == tr.OP_EVOLVE]
+    assert len(ev) > 0 and s.ops is s.ops
+    assert g == pytest.approx(float(np.sum(ev["t1"] - ev["t0"])), rel=1e-13, abs=0.0)
+    t = tr.Schedule(f, cosmo, tr.walk_params(T0))
+    t.close()
+    with pytest.raises(ValueError):
+        t.ops
+    with pytest.raises(Exception):
+        t.node_gyr()
+
+
 def test_walk_two_node_tree(cosmo):
     # closedBoxTree: node 1 (t=12.47) is the only progenitor of node 2 (t=13.47):
     # one interval to the parent's time, then the promotion.
