@@ -51,7 +51,7 @@ FP64_PEAK_TFLOPS = 78.6
 # counter scaled by what it reads for one elementwise kernel that reads and
 # writes 1 GiB (FETCH_SIZE x 2.0000, WRITE_SIZE x 1.0000).
 HBM_TRAFFIC_1E5 = {
-    (64, 1, 2): (241539.5 * 2.0 + 226578.7 * 1.0) * 1024,
+    (64, 1, 2): (241314.75 * 2.0 + 223554.75 * 1.0) * 1024,
     (64, 2, 2): (1922487.5 * 2.0 + 3820465.0 * 1.0) * 1024,
 }
 
